@@ -1,0 +1,172 @@
+"""Benchmark of BICOS::match on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one BICOS::match call over one synthetic 1920x1200, 33-frame uint16
+stereo pair (LIMITED descriptors, min_variance 2.0, nxcorr_threshold 0.96,
+subpixel_step 0.1, NoDuplicates) whose stacks are already resident in HBM.
+Prints one JSON line: throughput in matched Mpixel/s, the roofline position of
+the dominant kernel (per-kernel times from HIP events recorded by the library
+on the launch stream during the timed steps), and the CPU oracle timed on the
+host on the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "matched Mpixel/s @1920x1200x33 u16; learned vs plain at >=99.5% px agree"
+
+# MI355X published peaks
+HBM_PEAK_GBS = 8000.0
+INT8_DENSE_TOPS = 5000.0
+
+
+def parse_args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--seed", type=int, default=0x600DF00D)
+    return p.parse_args()
+
+
+def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int):
+    """Algorithmic work of one launch of `name` against the MI355X roof.
+
+    search: W*W*H candidates x (descriptor bits) AND-popcount pairs, counted
+    as int8 multiply-accumulates (2 ops each) against the dense int8 matrix
+    peak; transform / agree: bytes the algorithm must move (samples read,
+    outputs written) against HBM bandwidth."""
+    s = avg_ms * 1e-3
+    if name.startswith("search"):
+        ops = 2.0 * W * W * H * 32 * words
+        return {"bound": "mfma", "achieved": ops / s / 1e12, "peak": INT8_DENSE_TOPS,
+                "unit": "TFLOP/s", "frac": ops / s / 1e12 / INT8_DENSE_TOPS,
+                "work_per_launch": ops, "work_unit": "int8-equivalent ops"}
+    if name.startswith("transform"):
+        b = W * H * (n * 2 + 4 * words)
+    elif name.startswith("agree_subpixel"):
+        b = W * H * (2 + 4 * n * 2 + 4)      # disparity in, 4n samples, float out
+    else:
+        b = W * H * (2 + 2 * n * 2 + 2)
+    return {"bound": "hbm", "achieved": b / s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": b / s / 1e9 / HBM_PEAK_GBS, "work_per_launch": b, "work_unit": "bytes"}
+
+
+def committed_traffic(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by a copy-kernel calibration), recorded in profiles/."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\d+(?:\.\d+)?)", line)
+        if m and kernel.startswith(m.group(1)):
+            return float(m.group(2))
+    return None
+
+
+def cpu_baseline(L, R, cfg_kw):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the CPU oracle, timed as the baseline
+    threads = len(os.sched_getaffinity(0))
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    O.match(L, R, **cfg_kw)
+    dt = time.perf_counter() - t0
+    n, H, W = L.shape
+    return {"value": W * H / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"the full {W}x{H}x{n} workload once ({dt:.2f} s wall, "
+                      f"{threads} threads, oracle/bicos_oracle.cpp)"}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md on row-band sharding)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    import libbicos_amd as B
+    from libbicos_amd import scene
+
+    spec = scene.C1_1920x1200_U16
+    W, H, n = spec.width, spec.height, spec.frames
+    L, R, _ = scene.generate(spec, seed=args.seed)
+    cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
+                   mode=B.TransformMode.Limited, variant=B.Variant.NoDuplicates())
+    words = B.descriptor_bits(n, cfg.mode) // 32
+
+    dL = torch.from_numpy(L.view(np.int16)).cuda().view(torch.uint16)
+    dR = torch.from_numpy(R.view(np.int16)).cuda().view(torch.uint16)
+    stream = torch.cuda.current_stream()
+    disp, corr = B.match_device(dL, dR, cfg, stream=stream)
+    for _ in range(args.warmup):
+        disp, corr = B.match_device(dL, dR, cfg, disparity=disp, stream=stream)
+    torch.cuda.synchronize()
+
+    B.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        disp, corr = B.match_device(dL, dR, cfg, disparity=disp, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kstats = B.timing_read()
+    B.timing_enable(False)
+    ms_per_step = ev0.elapsed_time(ev1) / args.steps
+
+    kernels = {k: {"launches": int(c), "avg_ms": t / c} for k, (c, t) in kstats.items()}
+    per_step = {k: v["avg_ms"] * v["launches"] / args.steps for k, v in kernels.items()}
+    dom = max(per_step, key=per_step.get)
+    roof = kernel_roofline(dom, kernels[dom]["avg_ms"], W, H, n, words)
+    roof["kernel"] = dom
+    roof["traffic"] = committed_traffic(dom)
+    for k, v in kernels.items():
+        v["roofline_frac"] = round(kernel_roofline(k, v["avg_ms"], W, H, n, words)["frac"], 4)
+
+    d = disp.cpu().numpy()
+    valid = float(np.isfinite(d).mean())
+
+    out = {
+        "metric": METRIC,
+        "value": W * H / (ms_per_step * 1e-3) / 1e6,
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16 samples, u32 Hamming, f32 NCC",
+        "data": "synthetic",
+        "config": {"workload": f"plain BICOS::match {W}x{H}x{n} u16, LIMITED, min_variance 2.0, "
+                               "nxcorr_threshold 0.96, subpixel_step 0.1, NoDuplicates",
+                   "width": W, "height": H, "frames": n, "descriptor_bits": 32 * words,
+                   "valid_fraction": round(valid, 4), "wall_s_timed": round(wall, 4)},
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(L, R, dict(thr=0.96, step=0.1, min_variance=2.0,
+                                                      full=False))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * bicos_hip.h — C ABI of the MI355X (gfx950) BICOS matching library.
+ *
+ * This is the drop-in boundary for libBICOS's matching path. Every entry
+ * point takes plain pointers and sizes; device buffers are HIP device
+ * pointers, streams are hipStream_t passed as void*. Nothing here depends on
+ * OpenCV, torch or any C++ type. Calls are asynchronous on the given stream
+ * (NULL = the default stream): inputs must stay alive and outputs must not be
+ * read until the stream has reached the end of the call.
+ *
+ * Reference interfaces replaced (libBICOS v2.2.0):
+ *   bicos_hip_match            BICOS::match            include/match.hpp:31-41,
+ *                                                      src/lib.cpp:31-49,
+ *                                                      src/impl/cuda.cu.in:233-292
+ *   bicos_config               BICOS::Config           include/common.hpp:58-82
+ *   bicos_hip_transform        transform_{limited,full}_kernel
+ *                                                      include/impl/cuda/descriptor_transform.cuh:30-149
+ *   bicos_hip_search           bicos_kernel(_smem)     include/impl/cuda/bicos.cuh:77-176
+ *   bicos_hip_agree            agree_kernel            include/impl/cuda/agree.cuh:110-159
+ *   bicos_hip_agree_subpixel   agree_subpixel_kernel   include/impl/cuda/agree.cuh:161-259
+ *   error codes                BICOS::Exception / std::invalid_argument
+ *                                                      include/common.hpp:84-90, src/impl/cpu.cpp:151
+ */
+#ifndef BICOS_HIP_H
+#define BICOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element type codes: OpenCV's depth codes, kept so callers can pass
+ * cv::Mat::depth() straight through (include/common.hpp:50-56 uses cv::Mat). */
+enum {
+    BICOS_8U = 0,
+    BICOS_16U = 2,
+    BICOS_16S = 3,
+    BICOS_32F = 5,
+    BICOS_64F = 6
+};
+
+/* Return codes. BICOS_ERR_EXCEPTION corresponds to BICOS::Exception,
+ * BICOS_ERR_INVALID_ARGUMENT to std::invalid_argument in the reference. */
+enum {
+    BICOS_OK = 0,
+    BICOS_ERR_EXCEPTION = 1,
+    BICOS_ERR_INVALID_ARGUMENT = 2,
+    BICOS_ERR_HIP = 3
+};
+
+/* TransformMode (include/common.hpp:58) */
+enum { BICOS_MODE_LIMITED = 0, BICOS_MODE_FULL = 1 };
+
+/* Precision (include/common.hpp:59-61). BICOS_PRECISION_HALF_MFMA has no
+ * reference counterpart: NCC candidates scored in FP16 on the matrix cores,
+ * winners rescored in FP32 (not bit-exact; see the precision tests). */
+enum { BICOS_PRECISION_SINGLE = 0, BICOS_PRECISION_DOUBLE = 1, BICOS_PRECISION_HALF_MFMA = 2 };
+
+/* SearchVariant (include/common.hpp:63-71) */
+enum { BICOS_VARIANT_NO_DUPLICATES = 0, BICOS_VARIANT_CONSISTENCY = 1 };
+
+/* One single-channel image, device-resident (or host, where stated).
+ * Like cv::cuda::GpuMat: rows x cols elements of `type`, rows `step` bytes apart. */
+typedef struct bicos_image {
+    void* data;
+    int32_t rows;
+    int32_t cols;
+    int32_t type;   /* BICOS_8U, BICOS_16U, BICOS_16S, BICOS_32F, BICOS_64F */
+    int32_t _pad;
+    int64_t step;   /* bytes between rows */
+} bicos_image;
+
+/* BICOS::Config with explicit has_* flags for each std::optional.
+ * Reference defaults (include/common.hpp:73-82): has_nxcorr_threshold = 1,
+ * nxcorr_threshold = 0.5, no subpixel, no min_variance, LIMITED, SINGLE,
+ * NoDuplicates; Consistency defaults max_lr_diff = 1, no_dupes = 0. */
+typedef struct bicos_config {
+    int32_t has_nxcorr_threshold;
+    float nxcorr_threshold;
+    int32_t has_subpixel_step;
+    float subpixel_step;
+    int32_t has_min_variance;
+    float min_variance;
+    int32_t mode;        /* BICOS_MODE_* */
+    int32_t precision;   /* BICOS_PRECISION_* */
+    int32_t variant;     /* BICOS_VARIANT_* */
+    int32_t max_lr_diff; /* Consistency only */
+    int32_t no_dupes;    /* Consistency only */
+} bicos_config;
+
+/* Fills *cfg with the reference defaults. */
+void bicos_hip_config_default(bicos_config* cfg);
+
+/* Element type match() writes to `disparity` for this config: BICOS_16S, or
+ * BICOS_32F when both a threshold and a subpixel step are set
+ * (src/impl/cpu.cpp:85-88). */
+int bicos_hip_disparity_type(const bicos_config* cfg);
+
+/* Element type of the correlation map for this config (BICOS_32F, or
+ * BICOS_64F for BICOS_PRECISION_DOUBLE), or -1 when no threshold is set and
+ * no map is produced (src/impl/cuda.cu.in:183-191). */
+int bicos_hip_corrmap_type(const bicos_config* cfg);
+
+/* Descriptor width in bits chosen for n frames (32/64/128/256), or -1 when
+ * too many bits would be needed (src/impl/cpu.cpp:118-152). */
+int bicos_hip_descriptor_bits(uint32_t n, int32_t mode);
+
+/* BICOS::match. stack0/stack1: n device images each (BICOS_8U or BICOS_16U,
+ * all the same size and type). disparity: device image of the caller,
+ * rows x cols of bicos_hip_disparity_type(cfg). corrmap: NULL or a device
+ * image of bicos_hip_corrmap_type(cfg); it is filled with NaN and then
+ * written for every pixel whose correlation is evaluated (CPU semantics,
+ * src/impl/cpu.cpp:80-83). stream: hipStream_t or NULL. The library keeps a
+ * per-device workspace between calls (grown on demand, released by
+ * bicos_hip_release_workspace); calls on different streams are ordered
+ * through it. Returns BICOS_OK or an error code (bicos_hip_last_error()). */
+int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                    const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
+                    void* stream);
+
+/* Thread-local message of the last failed call on this thread. */
+const char* bicos_hip_last_error(void);
+
+/* Frees the cached workspace of the current device (waits for its last use). */
+int bicos_hip_release_workspace(void);
+
+/* ---- stage-level entry points (the reference's kernel-level tests drive
+ *      its kernels directly: test/descriptor_transform.cu, test/bicos.cu,
+ *      test/agree.cu) ---- */
+
+/* Descriptor transform of one view: frames = n device images (8U/16U),
+ * out = device buffer of rows*cols*words uint32 (bit k of a pixel's
+ * descriptor in word k/32, bit k%32), words = bits/32 for the width chosen
+ * by bicos_hip_descriptor_bits(n, mode) unless a wider one is asked for. */
+int bicos_hip_transform(const bicos_image* frames, uint32_t n, int32_t mode, int32_t words,
+                        uint32_t* out, void* stream);
+
+/* Row search on descriptor buffers (rows*cols*words uint32 each).
+ * flags: 1 = no duplicates, 2 = consistency (include/impl/common.hpp:46-47),
+ * 3 = both. out: device int16 rows*cols disparity (-32768 = invalid). */
+int bicos_hip_search(const uint32_t* desc0, const uint32_t* desc1, int32_t rows, int32_t cols,
+                     int32_t words, int32_t flags, int32_t max_lr_diff, int16_t* out,
+                     void* stream);
+
+/* In-place NCC agreement on an int16 disparity (device, rows*cols dense).
+ * min_variance is the per-frame value (it is multiplied by n here, like
+ * src/impl/cpu.cpp:123). corrmap: NULL or device rows*cols of float
+ * (precision SINGLE) or double (DOUBLE), written where evaluated. */
+int bicos_hip_agree(int16_t* disparity, const bicos_image* stack0, const bicos_image* stack1,
+                    uint32_t n, float nxcorr_threshold, int32_t has_min_variance,
+                    float min_variance, int32_t precision, void* corrmap, void* stream);
+
+/* Subpixel agreement: int16 disparity in, float disparity out (NaN =
+ * invalid); arguments as bicos_hip_agree. */
+int bicos_hip_agree_subpixel(const int16_t* disparity, const bicos_image* stack0,
+                             const bicos_image* stack1, uint32_t n, float nxcorr_threshold,
+                             float subpixel_step, int32_t has_min_variance, float min_variance,
+                             int32_t precision, float* out, void* corrmap, void* stream);
+
+/* ---- measurement ---- */
+
+/* Per-kernel GPU time accounting with HIP events recorded around every
+ * kernel the library launches (on the stream it launches on). */
+typedef struct bicos_kernel_stat {
+    char name[48];
+    int64_t launches;
+    double total_ms;
+} bicos_kernel_stat;
+
+int bicos_hip_timing_enable(int32_t enable);   /* resets the statistics */
+/* Waits for all recorded events, then copies up to cap entries; returns the
+ * number of distinct kernels (or a negative error code). */
+int bicos_hip_timing_read(bicos_kernel_stat* out, int32_t cap);
+
+/* ---- device memory helpers for callers without a HIP toolchain ---- */
+void* bicos_hip_malloc(size_t bytes);
+int bicos_hip_free(void* p);
+int bicos_hip_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int bicos_hip_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int bicos_hip_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BICOS_HIP_H */

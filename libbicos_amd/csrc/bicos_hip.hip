@@ -1,0 +1,648 @@
+// BICOS matching on MI355X (gfx950): host driver and C ABI.
+//
+// Pipeline of BICOS::match (reference: src/impl/cuda.cu.in:54-292,
+// src/impl/cpu.cpp:37-155):
+//   1. descriptor transform of both views        -> two descriptor planes
+//   2. Hamming row search (+ reverse pass for the Consistency variant)
+//   3. NCC agreement, or subpixel agreement when a step is set
+// All launches go to the caller's stream. Scratch memory (descriptor planes,
+// the int16 raw disparity of the subpixel path, the reverse-search columns)
+// lives in one per-device workspace kept between calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bicos_hip.h"
+#include "agree.hpp"
+#include "search.hpp"
+#include "transform.hpp"
+
+using namespace bicos_hip;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(BICOS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+/* ----------------------------------------------------------------- timing */
+
+struct TimedLaunch {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::vector<TimedLaunch> g_timed;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pooled_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+template<typename Kernel, typename... Args>
+int launch(const char* name, Kernel kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+           Args... args) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_timing) {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        e0 = pooled_event();
+        e1 = pooled_event();
+        if (e0) (void)hipEventRecord(e0, s);
+    }
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+        return fail(BICOS_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(err));
+    if (e0 && e1) {
+        (void)hipEventRecord(e1, s);
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        g_timed.push_back({name, e0, e1});
+    }
+    return BICOS_OK;
+}
+
+#define LAUNCH(...)                                \
+    do {                                           \
+        int rc_ = launch(__VA_ARGS__);             \
+        if (rc_ != BICOS_OK) return rc_;           \
+    } while (0)
+
+/* -------------------------------------------------------------- workspace */
+
+struct Workspace {
+    void* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t last = nullptr;
+    bool used = false;
+};
+
+std::mutex g_ws_mu;
+std::map<int, Workspace> g_ws;
+
+// Returns a device buffer of at least `bytes` that is free for use on `s`
+// (the stream waits for the previous call that used it). Caller holds g_ws_mu.
+int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out) {
+    Workspace& w = g_ws[dev];
+    if (!w.last) HIP_TRY(hipEventCreateWithFlags(&w.last, hipEventDisableTiming));
+    if (w.cap < bytes) {
+        if (w.used) HIP_TRY(hipEventSynchronize(w.last));
+        if (w.buf) HIP_TRY(hipFree(w.buf));
+        w.buf = nullptr;
+        w.cap = 0;
+        size_t want = bytes + bytes / 8;
+        HIP_TRY(hipMalloc(&w.buf, want));
+        w.cap = want;
+        w.used = false;
+    }
+    if (w.used) HIP_TRY(hipStreamWaitEvent(s, w.last, 0));
+    *out = w.buf;
+    return BICOS_OK;
+}
+
+int ws_release(int dev, hipStream_t s) {
+    Workspace& w = g_ws[dev];
+    HIP_TRY(hipEventRecord(w.last, s));
+    w.used = true;
+    return BICOS_OK;
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+/* ------------------------------------------------------------- validation */
+
+int elem_size(int type) {
+    switch (type) {
+        case BICOS_8U: return 1;
+        case BICOS_16U: return 2;
+        case BICOS_16S: return 2;
+        case BICOS_32F: return 4;
+        case BICOS_64F: return 8;
+        default: return 0;
+    }
+}
+
+int required_bits(uint32_t n, int mode) {
+    // src/impl/cpu.cpp:118-120
+    return mode == BICOS_MODE_FULL ? (int)(n * n - 2 * n + 3) : (int)(4 * n - 7);
+}
+
+int words_for_bits(int bits) {
+    // src/impl/cpu.cpp:125-152
+    if (bits <= 32) return 1;
+    if (bits <= 64) return 2;
+    if (bits <= 128) return 4;
+    if (bits <= 256) return 8;
+    return 0;
+}
+
+struct StackInfo {
+    int rows = 0, cols = 0, depth = 0;
+};
+
+int check_stack(const bicos_image* s, uint32_t n, StackInfo* info, FrameTable* ft) {
+    if (!s) return fail(BICOS_ERR_EXCEPTION, "null image stack");
+    if (n < 2) return fail(BICOS_ERR_EXCEPTION, "need at least two images");  // cpu.cpp:106
+    if (n > (uint32_t)MAX_FRAMES)
+        return fail(BICOS_ERR_EXCEPTION, "Bad number of images");  // cuda.cu.in:105
+    const int type = s[0].type;
+    if (type != BICOS_8U && type != BICOS_16U)
+        return fail(BICOS_ERR_EXCEPTION,
+                    "bad input depths, only CV_8UC1 and CV_16UC1 are supported");  // cpu.cpp:109
+    memset(ft, 0, sizeof(*ft));
+    for (uint32_t t = 0; t < n; ++t) {
+        if (s[t].type != type || s[t].rows != s[0].rows || s[t].cols != s[0].cols)
+            return fail(BICOS_ERR_EXCEPTION, "all images of a stack must share size and type");
+        if (!s[t].data || s[t].step < (int64_t)s[t].cols * elem_size(type))
+            return fail(BICOS_ERR_EXCEPTION, "bad image data pointer or step");
+        ft->ptr[t] = (uint64_t)s[t].data;
+        ft->step[t] = s[t].step;
+    }
+    info->rows = s[0].rows;
+    info->cols = s[0].cols;
+    info->depth = type;
+    if (info->rows < 1 || info->cols < 1) return fail(BICOS_ERR_EXCEPTION, "empty images");
+    if (info->cols > 32767)
+        return fail(BICOS_ERR_EXCEPTION, "image width above 32767 does not fit int16 disparities");
+    return BICOS_OK;
+}
+
+/* --------------------------------------------------------------- dispatch */
+
+template<typename TIn>
+int run_transform(const FrameTable& f, uint32_t n, int mode, int W, int H, int words,
+                  uint32_t* out, hipStream_t s) {
+    const dim3 grid((W + 255) / 256, H), block(256);
+    if (mode == BICOS_MODE_LIMITED) {
+        const int natural = words_for_bits(required_bits(n, mode));
+        switch (natural) {
+            case 1: LAUNCH("transform_limited_32", (k_transform_limited<TIn, 1, 9>), grid, block, 0, s, f, (int)n, W, words, out); break;
+            case 2: LAUNCH("transform_limited_64", (k_transform_limited<TIn, 2, 17>), grid, block, 0, s, f, (int)n, W, words, out); break;
+            case 4: LAUNCH("transform_limited_128", (k_transform_limited<TIn, 4, 33>), grid, block, 0, s, f, (int)n, W, words, out); break;
+            case 8: LAUNCH("transform_limited_256", (k_transform_limited<TIn, 8, 65>), grid, block, 0, s, f, (int)n, W, words, out); break;
+            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+        }
+    } else {
+        switch (n) {
+#define TF(N) case N: LAUNCH("transform_full", (k_transform_full<TIn, N>), grid, block, 0, s, f, W, words, out); break;
+            TF(2) TF(3) TF(4) TF(5) TF(6) TF(7) TF(8) TF(9) TF(10) TF(11) TF(12) TF(13) TF(14) TF(15) TF(16)
+#undef TF
+            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+        }
+    }
+    return BICOS_OK;
+}
+
+int transform_any(const FrameTable& f, int depth, uint32_t n, int mode, int W, int H, int words,
+                  uint32_t* out, hipStream_t s) {
+    return depth == BICOS_8U ? run_transform<uint8_t>(f, n, mode, W, H, words, out, s)
+                             : run_transform<uint16_t>(f, n, mode, W, H, words, out, s);
+}
+
+constexpr size_t SEARCH_LDS_BYTES = 64 * 1024;
+
+template<int WORDS, int R>
+int run_search_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes, int mode,
+                 const int16_t* rev, int max_lr, int16_t* out, int64_t ostep, hipStream_t s) {
+    const int chunk = (int)std::min<size_t>((size_t)W, SEARCH_LDS_BYTES / (WORDS * 4));
+    const size_t lds = (size_t)chunk * WORDS * 4;
+    const dim3 grid((W + SEARCH_THREADS * R - 1) / (SEARCH_THREADS * R), H), block(SEARCH_THREADS);
+#define SK(ND, M) LAUNCH("search", (k_search<WORDS, ND, M, R>), grid, block, lds, s, dq, dc, W, chunk, rev, max_lr, out, ostep)
+    if (nodupes) {
+        if (mode == SEARCH_DISP) SK(true, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SK(true, SEARCH_REV);
+        else SK(true, SEARCH_LR);
+    } else {
+        if (mode == SEARCH_DISP) SK(false, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SK(false, SEARCH_REV);
+        else SK(false, SEARCH_LR);
+    }
+#undef SK
+    return BICOS_OK;
+}
+
+int run_search(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
+               int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
+               hipStream_t s) {
+    switch (words) {
+        case 1: return run_search_w<1, 16>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+        case 2: return run_search_w<2, 8>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+        case 4: return run_search_w<4, 8>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+        case 8: return run_search_w<8, 4>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+        default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+    }
+}
+
+// flags: bit 0 NODUPES, bit 1 CONSISTENCY (include/impl/common.hpp:46-47)
+int search_flags(const uint32_t* d0, const uint32_t* d1, int H, int W, int words, int flags,
+                 int max_lr, int16_t* out, int64_t ostep, int16_t* rev_scratch, hipStream_t s) {
+    const bool nodupes = flags & 1;
+    if (flags & 2) {
+        int rc = run_search(d1, d0, H, W, words, nodupes, SEARCH_REV, nullptr, 0, rev_scratch, W, s);
+        if (rc) return rc;
+        return run_search(d0, d1, H, W, words, nodupes, SEARCH_LR, rev_scratch, max_lr, out, ostep, s);
+    }
+    return run_search(d0, d1, H, W, words, nodupes, SEARCH_DISP, nullptr, 0, out, ostep, s);
+}
+
+// stack depth buckets of the NCC kernels (registers per lane scale with NMAX)
+template<typename F>
+int with_nmax(uint32_t n, F&& f) {
+    if (n == 12) return f(std::integral_constant<int, 12>{}, std::integral_constant<int, 12>{});
+    if (n == 33) return f(std::integral_constant<int, 33>{}, std::integral_constant<int, 33>{});
+    if (n == 64) return f(std::integral_constant<int, 64>{}, std::integral_constant<int, 64>{});
+    if (n <= 8) return f(std::integral_constant<int, 8>{}, std::integral_constant<int, 0>{});
+    if (n <= 16) return f(std::integral_constant<int, 16>{}, std::integral_constant<int, 0>{});
+    if (n <= 24) return f(std::integral_constant<int, 24>{}, std::integral_constant<int, 0>{});
+    if (n <= 33) return f(std::integral_constant<int, 33>{}, std::integral_constant<int, 0>{});
+    if (n <= 48) return f(std::integral_constant<int, 48>{}, std::integral_constant<int, 0>{});
+    return f(std::integral_constant<int, 65>{}, std::integral_constant<int, 0>{});
+}
+
+template<typename TIn, typename TPrec>
+int run_agree_t(int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1,
+                uint32_t n, int W, int H, float thr, int has_mv, float mv, TPrec* corr,
+                int64_t cstep, hipStream_t s) {
+    const dim3 grid((W + 255) / 256, H), block(256);
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        LAUNCH("agree", (k_agree<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), grid, block, 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, has_mv, mv, corr, cstep);
+        return BICOS_OK;
+    });
+}
+
+template<typename TIn, typename TPrec>
+int run_subpixel_t(const int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1,
+                   uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
+                   float* out, int64_t ostep, TPrec* corr, int64_t cstep, hipStream_t s) {
+    const dim3 grid((W + 255) / 256, H), block(256);
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        LAUNCH("agree_subpixel", (k_agree_subpixel<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), grid, block, 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep);
+        return BICOS_OK;
+    });
+}
+
+int run_agree(int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1, int depth,
+              uint32_t n, int W, int H, float thr, int has_mv, float mv_n, int precision,
+              void* corr, int64_t cstep, hipStream_t s) {
+    if (precision == BICOS_PRECISION_DOUBLE) {
+        return depth == BICOS_8U
+            ? run_agree_t<uint8_t, double>(disp, dstep, f0, f1, n, W, H, thr, has_mv, mv_n, (double*)corr, cstep, s)
+            : run_agree_t<uint16_t, double>(disp, dstep, f0, f1, n, W, H, thr, has_mv, mv_n, (double*)corr, cstep, s);
+    }
+    return depth == BICOS_8U
+        ? run_agree_t<uint8_t, float>(disp, dstep, f0, f1, n, W, H, thr, has_mv, mv_n, (float*)corr, cstep, s)
+        : run_agree_t<uint16_t, float>(disp, dstep, f0, f1, n, W, H, thr, has_mv, mv_n, (float*)corr, cstep, s);
+}
+
+int run_subpixel(const int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1,
+                 int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                 float mv_n, int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
+                 hipStream_t s) {
+    if (precision == BICOS_PRECISION_DOUBLE) {
+        return depth == BICOS_8U
+            ? run_subpixel_t<uint8_t, double>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, s)
+            : run_subpixel_t<uint16_t, double>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, s);
+    }
+    return depth == BICOS_8U
+        ? run_subpixel_t<uint8_t, float>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s)
+        : run_subpixel_t<uint16_t, float>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s);
+}
+
+// Number of points of the subpixel grid `for (float x = -1; x <= 1; x += step)`
+// (agree.hpp:162), counted with the same float arithmetic the kernel uses;
+// -1 when the step is not positive or so small that the loop would not end
+// (the reference would spin forever) or would exceed 65536 points.
+int subpixel_points(float step) {
+    if (!(step > 0.0f)) return -1;
+    int k = 0;
+    for (float x = -1.0f; x <= 1.0f; ++k) {
+        const float nx = x + step;
+        if (!(nx > x) || k > 65536) return -1;
+        x = nx;
+    }
+    return k;
+}
+
+int check_step(float step) {
+    if (subpixel_points(step) < 0)
+        return fail(BICOS_ERR_INVALID_ARGUMENT,
+                    "subpixel_step must be positive and give at most 65536 grid points");
+    return BICOS_OK;
+}
+
+int check_out(const bicos_image* img, int rows, int cols, int type, const char* what) {
+    if (!img || !img->data)
+        return fail(BICOS_ERR_EXCEPTION, std::string(what) + ": null output image");
+    if (img->rows != rows || img->cols != cols)
+        return fail(BICOS_ERR_EXCEPTION, std::string(what) + ": size does not match the input");
+    if (img->type != type)
+        return fail(BICOS_ERR_EXCEPTION, std::string(what) + ": wrong element type");
+    const int es = elem_size(type);
+    if (img->step % es != 0 || img->step < (int64_t)cols * es)
+        return fail(BICOS_ERR_EXCEPTION, std::string(what) + ": bad row step");
+    return BICOS_OK;
+}
+
+int current_device(int* dev) {
+    HIP_TRY(hipGetDevice(dev));
+    return BICOS_OK;
+}
+
+}  // namespace
+
+/* ================================================================== C ABI */
+
+extern "C" {
+
+void bicos_hip_config_default(bicos_config* c) {
+    // include/common.hpp:73-82, include/common.hpp:66-69
+    memset(c, 0, sizeof(*c));
+    c->has_nxcorr_threshold = 1;
+    c->nxcorr_threshold = 0.5f;
+    c->mode = BICOS_MODE_LIMITED;
+    c->precision = BICOS_PRECISION_SINGLE;
+    c->variant = BICOS_VARIANT_NO_DUPLICATES;
+    c->max_lr_diff = 1;
+    c->no_dupes = 0;
+}
+
+int bicos_hip_disparity_type(const bicos_config* c) {
+    return (c->has_nxcorr_threshold && c->has_subpixel_step) ? BICOS_32F : BICOS_16S;
+}
+
+int bicos_hip_corrmap_type(const bicos_config* c) {
+    if (!c->has_nxcorr_threshold) return -1;
+    return c->precision == BICOS_PRECISION_DOUBLE ? BICOS_64F : BICOS_32F;
+}
+
+int bicos_hip_descriptor_bits(uint32_t n, int32_t mode) {
+    if (n < 2) return -1;
+    const int w = words_for_bits(required_bits(n, mode));
+    return w ? 32 * w : -1;
+}
+
+const char* bicos_hip_last_error(void) { return g_error.c_str(); }
+
+int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                    const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
+                    void* stream) {
+    if (!cfg) return fail(BICOS_ERR_INVALID_ARGUMENT, "null config");
+    hipStream_t s = (hipStream_t)stream;
+    StackInfo i0, i1;
+    FrameTable f0, f1;
+    int rc;
+    if ((rc = check_stack(stack0, n, &i0, &f0))) return rc;
+    if ((rc = check_stack(stack1, n, &i1, &f1))) return rc;
+    if (i0.rows != i1.rows || i0.cols != i1.cols || i0.depth != i1.depth)
+        return fail(BICOS_ERR_EXCEPTION, "the two stacks differ in size or type");
+    if (cfg->mode != BICOS_MODE_LIMITED && cfg->mode != BICOS_MODE_FULL)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown transform mode");
+    if (cfg->variant != BICOS_VARIANT_NO_DUPLICATES && cfg->variant != BICOS_VARIANT_CONSISTENCY)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented");  // cuda.cu.in:51
+    if (cfg->precision != BICOS_PRECISION_SINGLE && cfg->precision != BICOS_PRECISION_DOUBLE)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
+
+    const int bits = required_bits(n, cfg->mode);
+    const int words = words_for_bits(bits);
+    if (!words)
+        return fail(BICOS_ERR_INVALID_ARGUMENT,
+                    "input stacks too large, would require " + std::to_string(bits) + " bits");
+    const bool thr = cfg->has_nxcorr_threshold != 0;
+    const bool subpix = thr && cfg->has_subpixel_step;
+    if (subpix && (rc = check_step(cfg->subpixel_step))) return rc;
+
+    const int H = i0.rows, W = i0.cols;
+    if ((rc = check_out(disparity, H, W, bicos_hip_disparity_type(cfg), "disparity"))) return rc;
+    if (corrmap && thr && (rc = check_out(corrmap, H, W, bicos_hip_corrmap_type(cfg), "corrmap")))
+        return rc;
+
+    const bool consistency = cfg->variant == BICOS_VARIANT_CONSISTENCY;
+    const size_t npx = (size_t)W * H;
+    const size_t desc_bytes = align_up(npx * words * 4, 256);
+    const size_t raw_bytes = subpix ? align_up(npx * 2, 256) : 0;
+    const size_t rev_bytes = consistency ? align_up(npx * 2, 256) : 0;
+
+    int dev;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    void* ws;
+    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + rev_bytes, s, &ws))) return rc;
+    uint8_t* p = (uint8_t*)ws;
+    uint32_t* desc0 = (uint32_t*)p;
+    uint32_t* desc1 = (uint32_t*)(p + desc_bytes);
+    int16_t* raw = subpix ? (int16_t*)(p + 2 * desc_bytes) : nullptr;
+    int16_t* rev = consistency ? (int16_t*)(p + 2 * desc_bytes + raw_bytes) : nullptr;
+
+    auto body = [&]() -> int {
+        int r;
+        if ((r = transform_any(f0, i0.depth, n, cfg->mode, W, H, words, desc0, s))) return r;
+        if ((r = transform_any(f1, i0.depth, n, cfg->mode, W, H, words, desc1, s))) return r;
+
+        int flags = 1, lr = -1;  // cpu.cpp:70-77
+        if (consistency) {
+            flags = 2 | (cfg->no_dupes ? 1 : 0);
+            lr = cfg->max_lr_diff;
+        }
+        int16_t* d16 = subpix ? raw : (int16_t*)disparity->data;
+        const int64_t d16_step = subpix ? W : disparity->step / 2;
+        if ((r = search_flags(desc0, desc1, H, W, words, flags, lr, d16, d16_step, rev, s))) return r;
+        if (!thr) return BICOS_OK;
+
+        const float mv_n = cfg->has_min_variance ? cfg->min_variance * (float)n : 0.0f;  // cpu.cpp:123
+        void* cm = (corrmap && thr) ? corrmap->data : nullptr;
+        const int64_t cstep = cm ? corrmap->step / elem_size(corrmap->type) : 0;
+        if (subpix)
+            return run_subpixel(raw, W, f0, f1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                                cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision,
+                                (float*)disparity->data, disparity->step / 4, cm, cstep, s);
+        return run_agree(d16, d16_step, f0, f1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                         cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, s);
+    };
+    rc = body();
+    int rc2 = ws_release(dev, s);
+    return rc ? rc : rc2;
+}
+
+int bicos_hip_release_workspace(void) {
+    int dev, rc;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(dev);
+    if (it == g_ws.end()) return BICOS_OK;
+    Workspace& w = it->second;
+    if (w.used) HIP_TRY(hipEventSynchronize(w.last));
+    if (w.buf) HIP_TRY(hipFree(w.buf));
+    if (w.last) HIP_TRY(hipEventDestroy(w.last));
+    g_ws.erase(it);
+    return BICOS_OK;
+}
+
+int bicos_hip_transform(const bicos_image* frames, uint32_t n, int32_t mode, int32_t words,
+                        uint32_t* out, void* stream) {
+    StackInfo info;
+    FrameTable f;
+    int rc;
+    if ((rc = check_stack(frames, n, &info, &f))) return rc;
+    if (mode != BICOS_MODE_LIMITED && mode != BICOS_MODE_FULL)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown transform mode");
+    const int bits = required_bits(n, mode);
+    const int natural = words_for_bits(bits);
+    if (!natural)
+        return fail(BICOS_ERR_INVALID_ARGUMENT,
+                    "input stacks too large, would require " + std::to_string(bits) + " bits");
+    if (words != 1 && words != 2 && words != 4 && words != 8)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+    if (words < natural)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor too narrow for this stack");
+    if (!out) return fail(BICOS_ERR_EXCEPTION, "null output");
+    return transform_any(f, info.depth, n, mode, info.cols, info.rows, words, out,
+                         (hipStream_t)stream);
+}
+
+int bicos_hip_search(const uint32_t* desc0, const uint32_t* desc1, int32_t rows, int32_t cols,
+                     int32_t words, int32_t flags, int32_t max_lr_diff, int16_t* out,
+                     void* stream) {
+    if (!desc0 || !desc1 || !out) return fail(BICOS_ERR_EXCEPTION, "null buffer");
+    if (rows < 1 || cols < 1 || cols > 32767) return fail(BICOS_ERR_EXCEPTION, "bad size");
+    if (flags < 0 || flags > 3) return fail(BICOS_ERR_INVALID_ARGUMENT, "bad flags");
+    hipStream_t s = (hipStream_t)stream;
+    if (!(flags & 2))
+        return search_flags(desc0, desc1, rows, cols, words, flags, max_lr_diff, out, cols,
+                            nullptr, s);
+    int dev, rc;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    void* ws;
+    if ((rc = ws_acquire(dev, align_up((size_t)rows * cols * 2, 256), s, &ws))) return rc;
+    rc = search_flags(desc0, desc1, rows, cols, words, flags, max_lr_diff, out, cols,
+                      (int16_t*)ws, s);
+    int rc2 = ws_release(dev, s);
+    return rc ? rc : rc2;
+}
+
+int bicos_hip_agree(int16_t* disparity, const bicos_image* stack0, const bicos_image* stack1,
+                    uint32_t n, float thr, int32_t has_min_variance, float min_variance,
+                    int32_t precision, void* corrmap, void* stream) {
+    StackInfo i0, i1;
+    FrameTable f0, f1;
+    int rc;
+    if ((rc = check_stack(stack0, n, &i0, &f0))) return rc;
+    if ((rc = check_stack(stack1, n, &i1, &f1))) return rc;
+    if (!disparity) return fail(BICOS_ERR_EXCEPTION, "null disparity");
+    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
+    const float mv_n = has_min_variance ? min_variance * (float)n : 0.0f;
+    return run_agree(disparity, i0.cols, f0, f1, i0.depth, n, i0.cols, i0.rows, thr,
+                     has_min_variance, mv_n, precision, corrmap, i0.cols, (hipStream_t)stream);
+}
+
+int bicos_hip_agree_subpixel(const int16_t* disparity, const bicos_image* stack0,
+                             const bicos_image* stack1, uint32_t n, float thr,
+                             float subpixel_step, int32_t has_min_variance, float min_variance,
+                             int32_t precision, float* out, void* corrmap, void* stream) {
+    StackInfo i0, i1;
+    FrameTable f0, f1;
+    int rc;
+    if ((rc = check_stack(stack0, n, &i0, &f0))) return rc;
+    if ((rc = check_stack(stack1, n, &i1, &f1))) return rc;
+    if (!disparity || !out) return fail(BICOS_ERR_EXCEPTION, "null buffer");
+    if ((rc = check_step(subpixel_step))) return rc;
+    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
+    const float mv_n = has_min_variance ? min_variance * (float)n : 0.0f;
+    return run_subpixel(disparity, i0.cols, f0, f1, i0.depth, n, i0.cols, i0.rows, thr,
+                        subpixel_step, has_min_variance, mv_n, precision, out, i0.cols, corrmap,
+                        i0.cols, (hipStream_t)stream);
+}
+
+int bicos_hip_timing_enable(int32_t enable) {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    for (auto& t : g_timed) {
+        (void)hipEventSynchronize(t.stop);
+        g_event_pool.push_back(t.start);
+        g_event_pool.push_back(t.stop);
+    }
+    g_timed.clear();
+    g_timing = enable != 0;
+    return BICOS_OK;
+}
+
+int bicos_hip_timing_read(bicos_kernel_stat* out, int32_t cap) {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    std::vector<bicos_kernel_stat> stats;
+    for (auto& t : g_timed) {
+        HIP_TRY(hipEventSynchronize(t.stop));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, t.start, t.stop));
+        auto it = std::find_if(stats.begin(), stats.end(),
+                               [&](const bicos_kernel_stat& k) { return strcmp(k.name, t.name) == 0; });
+        if (it == stats.end()) {
+            bicos_kernel_stat k{};
+            strncpy(k.name, t.name, sizeof(k.name) - 1);
+            stats.push_back(k);
+            it = stats.end() - 1;
+        }
+        it->launches += 1;
+        it->total_ms += ms;
+    }
+    for (int i = 0; i < (int)stats.size() && i < cap; ++i) out[i] = stats[i];
+    return (int)stats.size();
+}
+
+void* bicos_hip_malloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        g_error = "hipMalloc failed";
+        return nullptr;
+    }
+    return p;
+}
+
+int bicos_hip_free(void* p) {
+    HIP_TRY(hipFree(p));
+    return BICOS_OK;
+}
+
+int bicos_hip_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return BICOS_OK;
+}
+
+int bicos_hip_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BICOS_OK;
+}
+
+int bicos_hip_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return BICOS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,349 @@
+"""HIP path vs the CPU oracle, through the C ABI, on the same seeded inputs.
+
+Mirrors the reference's differential tests (test/descriptor_transform.cu,
+test/bicos.cu, test/agree.cu, test/integration.cu): random inputs of random
+ragged sizes, every descriptor width, input depth, search variant and
+precision; plus whole-path runs on synthetic scenes of the benchmark
+workloads. The bar is bit-exact equality for descriptors, integer and float
+disparities and the correlation maps (NaN-aware); the FP32 NCC/subpixel
+tolerance of 1e-5 / 1e-3 px is asserted on top where it applies.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def dev(a):
+    import torch
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint16:
+        return torch.from_numpy(a.view(np.int16)).cuda().view(torch.uint16)
+    if a.dtype == np.uint32:
+        return torch.from_numpy(a.view(np.int32)).cuda()
+    return torch.from_numpy(a).cuda()
+
+
+def host(t, dtype=None):
+    import torch
+    torch.cuda.synchronize()
+    if t.dtype == torch.uint16:
+        return t.view(torch.int16).cpu().numpy().view(np.uint16)
+    a = t.cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def same(a, b):
+    """bitwise equality, NaN == NaN"""
+    if a.dtype.kind == "f":
+        return a.shape == b.shape and np.array_equal(a, b, equal_nan=True)
+    return np.array_equal(a, b)
+
+
+def rand_stack(rng, n, rows, cols, dtype, levels=None):
+    hi = levels if levels else np.iinfo(dtype).max + 1
+    return rng.integers(0, hi, size=(n, rows, cols)).astype(dtype)
+
+
+# ------------------------------------------------------------ transform
+
+LIMITED_N = [2, 3, 4, 5, 8, 9, 10, 16, 17, 18, 25, 32, 33, 34, 48, 64, 65]
+FULL_N = list(range(2, 17))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("full", [False, True])
+def test_transform_bit_exact(gpu, oracle, dtype, full):
+    rng = np.random.default_rng(100 + full + 2 * (dtype == np.uint16))
+    for n in (FULL_N if full else LIMITED_N):
+        rows, cols = int(rng.integers(8, 40)), int(rng.integers(256, 1028))
+        for levels in (None, 3):          # 3 levels: equal samples and equal sums are common
+            s = rand_stack(rng, n, rows, cols, dtype, levels)
+            exp = oracle.transform(s, full=full)
+            got = host(gpu.transform(dev(s), int(full)), np.uint32)
+            assert np.array_equal(got, exp), (n, levels)
+
+
+def test_transform_wider_than_needed(gpu, oracle):
+    rng = np.random.default_rng(3)
+    s = rand_stack(rng, 9, 16, 300, np.uint16)
+    for words in (1, 2, 4, 8):
+        got = host(gpu.transform(dev(s), 0, words=words), np.uint32)
+        assert np.array_equal(got, oracle.transform(s, full=False, words=words))
+
+
+def test_transform_pitched_frames(gpu, oracle):
+    """frames with a row pitch larger than the width (GpuMat-style padding)"""
+    rng = np.random.default_rng(4)
+    big = rand_stack(rng, 12, 40, 512, np.uint8)
+    s = big[:, :, :333]
+    d = dev(big)[:, :, :333]
+    got = host(gpu.transform(d, 1), np.uint32)
+    assert np.array_equal(got, oracle.transform(np.ascontiguousarray(s), full=True))
+
+
+# --------------------------------------------------------------- search
+
+@pytest.mark.parametrize("words", [1, 2, 4, 8])
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_search_exact(gpu, oracle, words, flags):
+    """test/bicos.cu: random descriptors, max_lr_diff 3, exact equality."""
+    rng = np.random.default_rng(1000 + 10 * words + flags)
+    rows, cols = int(rng.integers(16, 48)), int(rng.integers(256, 1028))
+    full = rng.integers(0, 2**32, size=(rows, cols, words), dtype=np.uint64).astype(np.uint32)
+    pool = rng.integers(0, 2**32, size=(64, words), dtype=np.uint64).astype(np.uint32)
+    pooled = pool[rng.integers(0, 64, size=(rows, cols))]
+    # low-entropy descriptors: many equal costs, exercising the tie rules
+    sparse = (full & rng.integers(0, 2**32, size=full.shape, dtype=np.uint64).astype(np.uint32)
+              & rng.integers(0, 2**32, size=full.shape, dtype=np.uint64).astype(np.uint32))
+    for d0, d1 in ((full, full[:, ::-1].copy()), (pooled, pool[rng.integers(0, 64, size=(rows, cols))]),
+                   (sparse, sparse[:, rng.permutation(cols)].copy())):
+        for lr in (0, 3):
+            exp = oracle.search(d0, d1, flags, lr)
+            got = host(gpu.search(dev(d0), dev(d1), flags, lr))
+            assert np.array_equal(got, exp), lr
+
+
+def test_search_chunked_rows(gpu, oracle):
+    """rows wider than one LDS chunk (64 KiB = 2048 x 256-bit descriptors)"""
+    rng = np.random.default_rng(9)
+    rows, cols, words = 3, 4500, 8
+    pool = rng.integers(0, 2**32, size=(300, words), dtype=np.uint64).astype(np.uint32)
+    d0 = pool[rng.integers(0, 300, size=(rows, cols))]
+    d1 = pool[rng.integers(0, 300, size=(rows, cols))]
+    for flags in (1, 2, 3):
+        assert np.array_equal(host(gpu.search(dev(d0), dev(d1), flags, 2)),
+                              oracle.search(d0, d1, flags, 2))
+
+
+def test_search_planted_duplicates(gpu, oracle):
+    cols = 700
+    rng = np.random.default_rng(5)
+    d1 = rng.integers(0, 2**32, size=(1, cols, 4), dtype=np.uint64).astype(np.uint32)
+    d0 = d1.copy()
+    d1[0, 650] = d1[0, 10]          # left pixel 10 now has two exact matches
+    got = host(gpu.search(dev(d0), dev(d1), 1))
+    assert got[0, 10] == -32768 and got[0, 11] == 0
+    assert np.array_equal(got, oracle.search(d0, d1, 1))
+    got0 = host(gpu.search(dev(d0), dev(d1), 0))
+    assert got0[0, 10] == 0
+
+
+# ---------------------------------------------------------------- agree
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("double", [False, True])
+def test_agree_exact(gpu, oracle, dtype, double):
+    """test/agree.cu: n = 10, random images and disparities in [-1, W),
+    random threshold and minimum variance; int16 output and corrmap exact."""
+    import torch
+    rng = np.random.default_rng(2000 + double + 2 * (dtype == np.uint16))
+    for n in (10, 12, 33, 64, 7):
+        rows, cols = 24, int(rng.integers(1024, 2048))
+        s0 = rand_stack(rng, n, rows, cols, dtype)
+        s1 = rand_stack(rng, n, rows, cols, dtype)
+        disp = rng.integers(-1, cols, size=(rows, cols)).astype(np.int16)
+        disp[rng.random((rows, cols)) < 0.1] = -32768
+        thr = float(rng.uniform(-0.9, 0.9))
+        for minvar in (float(rng.uniform(0, 75)), None):
+            mv_n = None if minvar is None else np.float32(minvar) * np.float32(n)
+            exp, exp_cm = oracle.agree(disp, s0, s1, thr, mv_n, double=double, corrmap=True)
+            d = dev(disp)
+            cm = torch.empty((rows, cols), dtype=torch.float64 if double else torch.float32,
+                             device="cuda")
+            gpu.agree(d, dev(s0), dev(s1), thr, minvar, 1 if double else 0, corrmap=cm)
+            assert np.array_equal(host(d), exp), (n, minvar)
+            assert same(host(cm), exp_cm), (n, minvar)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("double", [False, True])
+def test_agree_subpixel_exact(gpu, oracle, dtype, double):
+    import torch
+    rng = np.random.default_rng(3000 + double + 2 * (dtype == np.uint16))
+    for n, step in ((10, 0.25), (33, 0.1), (12, 0.05), (64, 0.05), (5, 0.3)):
+        rows, cols = 12, int(rng.integers(1024, 1600))
+        # smooth random rows so that interpolation sits near real correlation peaks
+        base = rand_stack(rng, n, rows, cols + 8, dtype)
+        s0 = base[:, :, 4:cols + 4].copy()
+        s1 = base[:, :, 2:cols + 2].copy()
+        disp = np.full((rows, cols), 2, np.int16)
+        mix = rng.random((rows, cols))
+        disp[mix < 0.3] = rng.integers(-1, cols, size=int((mix < 0.3).sum()))
+        disp[mix > 0.95] = -32768
+        disp[:, 0] = 0                       # col1 == 0 border path
+        disp[:, -1] = 0                      # col1 == W-1 border path
+        thr = float(rng.uniform(-0.5, 0.9))
+        for minvar in (float(rng.uniform(0, 75)), None):
+            mv_n = None if minvar is None else np.float32(minvar) * np.float32(n)
+            exp, exp_cm = oracle.agree_subpixel(disp, s0, s1, thr, step, mv_n, double=double,
+                                                corrmap=True)
+            cm = torch.empty((rows, cols), dtype=torch.float64 if double else torch.float32,
+                             device="cuda")
+            out = gpu.agree_subpixel(dev(disp), dev(s0), dev(s1), thr, step, minvar,
+                                     1 if double else 0, corrmap=cm)
+            got, got_cm = host(out), host(cm)
+            assert same(got, exp), (n, step, minvar)
+            assert same(got_cm, exp_cm), (n, step, minvar)
+
+
+# ----------------------------------------------------------- whole path
+
+def run_match(gpu, L, R, cfg):
+    d, cm = gpu.match_device(dev(L), dev(R), cfg, want_corrmap=True)
+    return host(d), (host(cm) if cm is not None else None)
+
+
+def oracle_match(oracle, L, R, cfg, gpu):
+    cons = None
+    if isinstance(cfg.variant, gpu.Variant.Consistency):
+        cons = (cfg.variant.max_lr_diff, cfg.variant.no_dupes)
+    return oracle.match(L, R, thr=cfg.nxcorr_threshold, step=cfg.subpixel_step,
+                        min_variance=cfg.min_variance, full=cfg.mode == 1,
+                        double=cfg.precision == 1, consistency=cons, corrmap=True)
+
+
+def test_match_c0_640x480_u8_full(gpu, oracle):
+    """640x480 u8, 12 frames, FULL, threshold 0.9, no subpixel, NoDuplicates."""
+    from libbicos_amd import scene
+    L, R, gt = scene.generate(scene.C0_640x480_U8, seed=101)
+    cfg = gpu.Config(nxcorr_threshold=0.9, mode=gpu.TransformMode.Full)
+    got, got_cm = run_match(gpu, L, R, cfg)
+    exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+    assert got.dtype == np.int16 and np.array_equal(got, exp)
+    assert same(got_cm, exp_cm)
+    v = got != -32768
+    assert (np.abs(got[v] - gt[v]) <= 1).mean() > 0.95
+
+
+def test_match_c1_band_u16_limited_subpixel(gpu, oracle):
+    """1920 wide u16, 33 frames, LIMITED, min_variance 2, threshold 0.96, step 0.1."""
+    from libbicos_amd import scene
+    L, R, gt = scene.generate(scene.C1_1920x1200_U16, seed=102, height=64)
+    cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    got, got_cm = run_match(gpu, L, R, cfg)
+    exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+    assert got.dtype == np.float32
+    assert same(got, exp) and same(got_cm, exp_cm)
+    v = ~np.isnan(got)
+    assert v.mean() > 0.5 and (np.abs(got[v] - gt[v]) <= 1).mean() > 0.95
+
+
+@pytest.mark.parametrize("thr", [0.5, 0.75, 0.9])
+@pytest.mark.parametrize("variant", ["nodup", "cons1", "cons2dup"])
+def test_match_integration_matrix(gpu, oracle, thr, variant):
+    """test/integration.cu:45-87: FULL mode on 16 frames (256-bit descriptors),
+    thresholds x variants x subpixel steps {none, 0.1, 0.25, 0.5}."""
+    from libbicos_amd import scene
+    spec = scene.SceneSpec(720, 48, 16, 8, 5.0, 90.0, 20.0, 235.0, 2.0)
+    L, R, _ = scene.generate(spec, seed=103)
+    var = {"nodup": gpu.Variant.NoDuplicates(), "cons1": gpu.Variant.Consistency(1, False),
+           "cons2dup": gpu.Variant.Consistency(2, True)}[variant]
+    for step in (None, 0.1, 0.25, 0.5):
+        cfg = gpu.Config(nxcorr_threshold=thr, subpixel_step=step, mode=gpu.TransformMode.Full,
+                         variant=var)
+        got, got_cm = run_match(gpu, L, R, cfg)
+        exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+        assert same(got, exp), step
+        assert same(got_cm, exp_cm), step
+
+
+def test_match_c4_band_u16_64frames_consistency(gpu, oracle):
+    """4096 wide u16, 64 frames (256-bit), Consistency(max_lr_diff 1), step 0.05."""
+    from libbicos_amd import scene
+    L, R, gt = scene.generate(scene.C4_4096x3000_U16, seed=104, height=6)
+    cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.05, min_variance=2.0,
+                     variant=gpu.Variant.Consistency(1, False))
+    got, got_cm = run_match(gpu, L, R, cfg)
+    exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+    assert same(got, exp) and same(got_cm, exp_cm)
+    v = ~np.isnan(got)
+    assert v.mean() > 0.5 and (np.abs(got[v] - gt[v]) <= 1).mean() > 0.95
+
+
+def test_match_precision_double_and_no_threshold(gpu, oracle):
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=105, height=16, width=800, frames=20)
+    for cfg in (gpu.Config(nxcorr_threshold=0.8, subpixel_step=0.2, precision=gpu.Precision.Double),
+                gpu.Config(nxcorr_threshold=0.8, precision=gpu.Precision.Double, min_variance=5.0),
+                gpu.Config(nxcorr_threshold=None),
+                gpu.Config(nxcorr_threshold=None, subpixel_step=0.1)):
+        got, got_cm = run_match(gpu, L, R, cfg)
+        exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+        assert same(got, exp)
+        assert (got_cm is None) == (exp_cm is None)
+        if got_cm is not None:
+            assert same(got_cm, exp_cm)
+
+
+def test_match_full_size_c1(gpu, oracle):
+    """The whole 1920x1200x33 u16 workload, bit-exact against the oracle."""
+    from libbicos_amd import scene
+    L, R, gt = scene.generate(scene.C1_1920x1200_U16, seed=scene.BENCH_SEED)
+    cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    got, got_cm = run_match(gpu, L, R, cfg)
+    exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+    assert same(got, exp) and same(got_cm, exp_cm)
+    v = ~np.isnan(got)
+    print(f"valid {v.mean():.4f}, within 1px of ground truth {(np.abs(got[v] - gt[v]) <= 1).mean():.4f}")
+
+
+def test_workspace_reuse_across_sizes(gpu, oracle):
+    """consecutive calls with shrinking and growing sizes reuse the workspace"""
+    from libbicos_amd import scene
+    cfg = gpu.Config(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
+    for rows, cols in ((40, 600), (8, 200), (60, 900), (8, 200)):
+        L, R, _ = scene.generate(scene.C0_640x480_U8, seed=rows * cols, height=rows, width=cols)
+        got, got_cm = run_match(gpu, L, R, cfg)
+        exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+        assert same(got, exp) and same(got_cm, exp_cm)
+
+
+def test_numpy_match_api(gpu, oracle):
+    """pybicos-style host API: lists of numpy images in, numpy out."""
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.C0_640x480_U8, seed=106, height=32, width=400)
+    cfg = gpu.Config(nxcorr_threshold=0.9, mode=gpu.TransformMode.Full)
+    d, cm = gpu.match(list(L), list(R), cfg)
+    exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+    assert np.array_equal(d, exp) and same(cm, exp_cm)
+    d2, cm2 = gpu.match(list(L), list(R), gpu.Config(nxcorr_threshold=None, mode=1))
+    assert cm2 is None and d2.dtype == np.int16
+
+
+def test_cpp_api_cli(gpu, oracle, tmp_path):
+    """BICOS::match through the C++ API (bicos_cli) with the reference CLI's
+    option mapping: FULL unless --limited, min variance default 1.0."""
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=107, height=20, width=640, frames=24)
+    (tmp_path / "l.raw").write_bytes(L.tobytes())
+    (tmp_path / "r.raw").write_bytes(R.tobytes())
+    cli = os.path.join(ROOT, "libbicos_amd", "bicos_cli")
+    out = tmp_path / "d.raw"
+    cmd = [cli, "--left", str(tmp_path / "l.raw"), "--right", str(tmp_path / "r.raw"),
+           "-n", "24", "--rows", "20", "--cols", "640", "--depth", "16", "--limited",
+           "-t", "0.9", "-s", "0.1", "-m", "2", "--corrmap", str(tmp_path / "c.raw"), "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.frombuffer(out.read_bytes(), np.float32).reshape(20, 640)
+    got_cm = np.frombuffer((tmp_path / "c.raw").read_bytes(), np.float32).reshape(20, 640)
+    exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.1, min_variance=1.0, full=False,
+                               consistency=(2, False), corrmap=True)
+    assert same(got, exp) and same(got_cm, exp_cm)
+
+
+def test_errors_on_device(gpu):
+    import torch
+    s = torch.zeros((17, 4, 8), dtype=torch.uint8, device="cuda")
+    with pytest.raises(ValueError, match="258 bits"):
+        gpu.match_device(s, s, gpu.Config(mode=gpu.TransformMode.Full))
+    with pytest.raises(gpu.Exception, match="at least two"):
+        gpu.match_device(s[:1], s[:1])
+    with pytest.raises(ValueError, match="subpixel_step"):
+        gpu.match_device(s[:4], s[:4], gpu.Config(subpixel_step=-0.1))
