@@ -34,6 +34,8 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=0x600DF00D)
+    p.add_argument("--search", choices=("mfma", "valu"), default="mfma",
+                   help="row-search kernel: int8 matrix cores (default) or vector ALU")
     return p.parse_args()
 
 
@@ -107,6 +109,7 @@ def main():
     cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
                    mode=B.TransformMode.Limited, variant=B.Variant.NoDuplicates())
     words = B.descriptor_bits(n, cfg.mode) // 32
+    B.set_search_impl(B.SEARCH_MFMA if args.search == "mfma" else B.SEARCH_VALU)
 
     dL = torch.from_numpy(L.view(np.int16)).cuda().view(torch.uint16)
     dR = torch.from_numpy(R.view(np.int16)).cuda().view(torch.uint16)
@@ -158,6 +161,7 @@ def main():
         "config": {"workload": f"plain BICOS::match {W}x{H}x{n} u16, LIMITED, min_variance 2.0, "
                                "nxcorr_threshold 0.96, subpixel_step 0.1, NoDuplicates",
                    "width": W, "height": H, "frames": n, "descriptor_bits": 32 * words,
+                   "search_kernel": args.search,
                    "valid_fraction": round(valid, 4), "wall_s_timed": round(wall, 4)},
         "roofline": roof,
         "kernels": kernels,

@@ -126,6 +126,15 @@ const char* bicos_hip_last_error(void);
 /* Frees the cached workspace of the current device (waits for its last use). */
 int bicos_hip_release_workspace(void);
 
+/* Selects the row-search kernel for subsequent calls of this process:
+ * BICOS_SEARCH_MFMA (default; Hamming costs from int8 matrix-core products)
+ * or BICOS_SEARCH_VALU (xor/popcount on the vector ALU). Both give identical
+ * results; the switch exists for testing and measurement. No reference
+ * counterpart (the reference picks between its shared-memory and
+ * global-memory search kernels by row size, src/impl/cuda.cu.in:149-177). */
+enum { BICOS_SEARCH_MFMA = 0, BICOS_SEARCH_VALU = 1 };
+int bicos_hip_set_search_impl(int32_t impl);
+
 /* ---- stage-level entry points (the reference's kernel-level tests drive
  *      its kernels directly: test/descriptor_transform.cu, test/bicos.cu,
  *      test/agree.cu) ---- */

@@ -1,6 +1,7 @@
 """Build recipe of the native parts of libbicos_amd.
 
-    python -m libbicos_amd.build        # or: from libbicos_amd.build import build; build()
+    python libbicos_amd/build.py        # (a plain script: it must not import the
+                                        #  package, whose import loads the library)
 
 Produces, in-tree next to this file:
   libbicos_hip.so    gfx950 kernels + C ABI (include/bicos_hip.h) + C++ API
@@ -31,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["bicos_hip.hip", "match.cpp"]
-HIP_DEPS = ["device_common.hpp", "transform.hpp", "search.hpp", "agree.hpp"]
+HIP_DEPS = ["device_common.hpp", "transform.hpp", "search.hpp", "search_mfma.hpp", "agree.hpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -22,6 +22,7 @@
 #include "../../include/bicos_hip.h"
 #include "agree.hpp"
 #include "search.hpp"
+#include "search_mfma.hpp"
 #include "transform.hpp"
 
 using namespace bicos_hip;
@@ -246,9 +247,42 @@ int run_search_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodu
     return BICOS_OK;
 }
 
+// Search implementation: matrix cores (default) or vector ALU.
+int g_search_impl = BICOS_SEARCH_MFMA;
+
+template<int WORDS, int NB, int WAVES, int CH>
+int run_search_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes,
+                      int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
+                      hipStream_t s) {
+    constexpr size_t lds = (size_t)CH * (2 * WORDS + 1) * 16;
+    static_assert(lds <= 64 * 1024, "LDS chunk too large");
+    const dim3 grid((W + WAVES * NB * 32 - 1) / (WAVES * NB * 32), H), block(WAVES * 64);
+#define SM(ND, M) LAUNCH("search_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep)
+    if (nodupes) {
+        if (mode == SEARCH_DISP) SM(true, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SM(true, SEARCH_REV);
+        else SM(true, SEARCH_LR);
+    } else {
+        if (mode == SEARCH_DISP) SM(false, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SM(false, SEARCH_REV);
+        else SM(false, SEARCH_LR);
+    }
+#undef SM
+    return BICOS_OK;
+}
+
 int run_search(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
                int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
                hipStream_t s) {
+    if (g_search_impl == BICOS_SEARCH_MFMA) {
+        switch (words) {
+            case 1: return run_search_mfma_w<1, 4, 8, 512>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 2: return run_search_mfma_w<2, 4, 8, 512>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 4: return run_search_mfma_w<4, 4, 8, 256>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 8: return run_search_mfma_w<8, 2, 8, 224>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+        }
+    }
     switch (words) {
         case 1: return run_search_w<1, 16>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
         case 2: return run_search_w<2, 8>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
@@ -488,6 +522,13 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     rc = body();
     int rc2 = ws_release(dev, s);
     return rc ? rc : rc2;
+}
+
+int bicos_hip_set_search_impl(int32_t impl) {
+    if (impl != BICOS_SEARCH_MFMA && impl != BICOS_SEARCH_VALU)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown search implementation");
+    g_search_impl = impl;
+    return BICOS_OK;
 }
 
 int bicos_hip_release_workspace(void) {

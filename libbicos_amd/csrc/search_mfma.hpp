@@ -1,0 +1,166 @@
+// Full-row Hamming argmin search on the gfx950 matrix cores.
+//
+// Same result as k_search (include/impl/cpu/bicos.hpp:50-113 semantics), but
+// the Hamming costs of a 32 x 32 block of (candidate, query) pairs come from
+// v_mfma_i32_32x32x32_i8 instead of xor/popcount:
+//
+//   query q (the pixel searched for)  -> int8 entries 1 - 2 q_k  (+1 / -1)
+//   candidate c                       -> int8 entries c_k        ( 0 / 1 )
+//   sum_k c_k (1 - 2 q_k) = popcount(c) - 2 popcount(c & q)
+//                         = popcount(c ^ q) - popcount(q)
+//
+// popcount(q) is constant per query, so the matrix product ranks candidates
+// exactly like the Hamming cost. Each output then costs three VALU ops:
+// key = (dot << 16) + column (v_lshl_add_u32), second-smallest key
+// (v_med3_u32, no-duplicates only) and smallest key (v_min_u32); keys are
+// unique per column, so the first strict minimum and the duplicate test come
+// out as in the reference scan, independent of evaluation order.
+//
+// Layout. A block serves WAVES*NB*32 query pixels of one image row; each
+// wave holds NB query tiles of 32 pixels as MFMA B operands in registers
+// (expanded once). The candidate row is expanded to one byte per descriptor
+// bit and staged in LDS in chunks of CH columns, padded by 16 bytes per
+// column so the 128-bit operand reads are bank-conflict free. Output lane
+// layout of the 32x32 MFMA: lane l, accumulator i holds query column
+// n = l % 32 and candidate row m = 8 (i / 4) + 4 (l / 32) + i % 4. Lanes l and
+// l ^ 32 see the same query and disjoint candidates; their minima are merged
+// at the end.
+#pragma once
+
+#include "device_common.hpp"
+#include "search.hpp"
+
+namespace bicos_hip {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// 4 bits -> 4 bytes of 0/1 (bit b to byte b)
+__device__ __forceinline__ uint32_t spread4(uint32_t x) {
+    return ((x & 0xFu) * 0x00204081u) & 0x01010101u;
+}
+
+// 16 descriptor bits -> 16 int8 entries of 0/1
+__device__ __forceinline__ uint4 expand01(uint32_t b16) {
+    return make_uint4(spread4(b16), spread4(b16 >> 4), spread4(b16 >> 8), spread4(b16 >> 12));
+}
+
+// 16 descriptor bits -> 16 int8 entries of +1 (bit 0) / -1 (bit 1)
+__device__ __forceinline__ v4i expand_pm1(uint32_t b16) {
+    v4i r;
+    r[0] = (int)(spread4(b16) * 0xFEu | 0x01010101u);
+    r[1] = (int)(spread4(b16 >> 4) * 0xFEu | 0x01010101u);
+    r[2] = (int)(spread4(b16 >> 8) * 0xFEu | 0x01010101u);
+    r[3] = (int)(spread4(b16 >> 12) * 0xFEu | 0x01010101u);
+    return r;
+}
+
+// "no candidate" key: above every real key (< 2^26) and far enough below 2^32
+// that the state offsets (at most 32 * 1024 tiles + 4) cannot wrap it around.
+constexpr uint32_t NO_KEY = 0x7FFFFFFFu;
+
+template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH>
+__global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
+    const uint32_t* __restrict__ dq, const uint32_t* __restrict__ dc, int W,
+    const int16_t* __restrict__ rev, int max_lr_diff, int16_t* __restrict__ out, int64_t ostep) {
+    constexpr int KS = WORDS;              // K steps of 32 bits
+    constexpr int STRIDE = 2 * KS + 1;     // uint4 per staged candidate (16 B pad)
+    extern __shared__ uint4 lds[];
+
+    const int row = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nl = lane & 31, g = lane >> 5;
+
+    // ---- query operands: bits 32 s + 16 g .. +15 of each query pixel
+    v4i qop[NB][KS];
+    uint32_t s1[NB], s2[NB];
+    const int qbase = (blockIdx.x * WAVES + wave) * (NB * 32);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int col = qbase + b * 32 + nl;
+        const uint32_t* src = dq + ((size_t)row * W + (col < W ? col : 0)) * WORDS;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) qop[b][s] = expand_pm1(src[s] >> (16 * g));
+        s1[b] = NO_KEY;
+        s2[b] = NO_KEY;
+    }
+
+    const uint32_t* crow = dc + (size_t)row * W * WORDS;
+    const int ntiles = (W + 31) / 32;
+    for (int c0 = 0; c0 < W; c0 += CH) {
+        const int cn = min(CH, W - c0);
+        const int cnp = (cn + 31) & ~31;
+        __syncthreads();
+        for (int it = threadIdx.x; it < cnp * 2 * KS; it += WAVES * 64) {
+            const int m = it / (2 * KS), c = it - m * (2 * KS);
+            uint32_t bits = 0;
+            if (m < cn) bits = (crow[(size_t)(c0 + m) * WORDS + (c >> 1)] >> (16 * (c & 1))) & 0xFFFFu;
+            lds[m * STRIDE + c] = expand01(bits);
+        }
+        __syncthreads();
+
+        for (int mt = 0; mt < cnp / 32; ++mt) {
+            v4i aop[KS];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const uint4 v = lds[(mt * 32 + nl) * STRIDE + 2 * s + g];
+                aop[s] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+            }
+            const int tbase = c0 + mt * 32;  // first candidate column of the tile
+            const bool partial = tbase + 32 > W;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                v16i acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], v16i{0}, 0, 0, 0);
+#pragma unroll
+                for (int s = 1; s < KS; ++s)
+                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[s], qop[b][s], acc, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    // candidate row within the tile: 8 (i / 4) + 4 g + i % 4; the
+                    // 4 g part and the tile base live in the running state offset
+                    const uint32_t ci = (256u << 16) + 8u * (i / 4) + (i % 4);
+                    uint32_t key = ((uint32_t)acc[i] << 16) + ci;
+                    if (partial && tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = NO_KEY;
+                    if constexpr (NODUPES) s2[b] = umed3(s1[b], s2[b], key);
+                    s1[b] = min(s1[b], key);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                s1[b] -= 32u;
+                if constexpr (NODUPES) s2[b] -= 32u;
+            }
+        }
+    }
+
+    // restore absolute keys: + 32 * (tiles scanned) + 4 g, then merge lanes l, l ^ 32
+    const uint32_t off = 32u * (uint32_t)ntiles + 4u * (uint32_t)g;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        uint32_t b1 = s1[b] + off, b2 = s2[b] + off;
+        const uint32_t o1 = (uint32_t)__shfl_xor((int)b1, 32);
+        const uint32_t o2 = (uint32_t)__shfl_xor((int)b2, 32);
+        const uint32_t m1 = min(b1, o1);
+        const uint32_t m2 = min(max(b1, o1), min(b2, o2));
+        const int col = qbase + b * 32 + nl;
+        if (g != 0 || col >= W)
+            continue;
+        const int best = (int)(m1 & 0xFFFFu);
+        const bool valid = !(NODUPES && (m1 >> 16) == (m2 >> 16));
+        int16_t v = INVALID16;
+        if (valid) {
+            if constexpr (MODE == SEARCH_DISP) {
+                v = (int16_t)(col - best);
+            } else if constexpr (MODE == SEARCH_REV) {
+                v = (int16_t)best;
+            } else {
+                const int r0 = rev[(size_t)row * W + best];
+                if (r0 != INVALID16 && abs(col - r0) <= max_lr_diff)
+                    v = (int16_t)((col + r0) / 2 - best);
+            }
+        }
+        out[(size_t)row * ostep + col] = v;
+    }
+}
+
+}  // namespace bicos_hip

@@ -90,12 +90,22 @@ def test_transform_pitched_frames(gpu, oracle):
 
 # --------------------------------------------------------------- search
 
+@pytest.fixture(params=["mfma", "valu"])
+def search_impl(request, gpu):
+    """both row-search kernels: int8 matrix cores (default) and vector ALU"""
+    gpu.set_search_impl(gpu.SEARCH_MFMA if request.param == "mfma" else gpu.SEARCH_VALU)
+    yield request.param
+    gpu.set_search_impl(gpu.SEARCH_MFMA)
+
+
 @pytest.mark.parametrize("words", [1, 2, 4, 8])
 @pytest.mark.parametrize("flags", [0, 1, 2, 3])
-def test_search_exact(gpu, oracle, words, flags):
+def test_search_exact(gpu, oracle, search_impl, words, flags):
     """test/bicos.cu: random descriptors, max_lr_diff 3, exact equality."""
     rng = np.random.default_rng(1000 + 10 * words + flags)
     rows, cols = int(rng.integers(16, 48)), int(rng.integers(256, 1028))
+    if flags == 1:
+        cols = [1, 5, 33, 300][words.bit_length() - 1]   # tiny and ragged rows
     full = rng.integers(0, 2**32, size=(rows, cols, words), dtype=np.uint64).astype(np.uint32)
     pool = rng.integers(0, 2**32, size=(64, words), dtype=np.uint64).astype(np.uint32)
     pooled = pool[rng.integers(0, 64, size=(rows, cols))]
@@ -110,8 +120,9 @@ def test_search_exact(gpu, oracle, words, flags):
             assert np.array_equal(got, exp), lr
 
 
-def test_search_chunked_rows(gpu, oracle):
-    """rows wider than one LDS chunk (64 KiB = 2048 x 256-bit descriptors)"""
+def test_search_chunked_rows(gpu, oracle, search_impl):
+    """rows wider than one LDS chunk (VALU: 2048 256-bit descriptors; MFMA:
+    224 expanded 256-bit descriptors), with a ragged last tile"""
     rng = np.random.default_rng(9)
     rows, cols, words = 3, 4500, 8
     pool = rng.integers(0, 2**32, size=(300, words), dtype=np.uint64).astype(np.uint32)
@@ -122,7 +133,7 @@ def test_search_chunked_rows(gpu, oracle):
                               oracle.search(d0, d1, flags, 2))
 
 
-def test_search_planted_duplicates(gpu, oracle):
+def test_search_planted_duplicates(gpu, oracle, search_impl):
     cols = 700
     rng = np.random.default_rng(5)
     d1 = rng.integers(0, 2**32, size=(1, cols, 4), dtype=np.uint64).astype(np.uint32)
@@ -254,7 +265,7 @@ def test_match_integration_matrix(gpu, oracle, thr, variant):
         assert same(got_cm, exp_cm), step
 
 
-def test_match_c4_band_u16_64frames_consistency(gpu, oracle):
+def test_match_c4_band_u16_64frames_consistency(gpu, oracle, search_impl):
     """4096 wide u16, 64 frames (256-bit), Consistency(max_lr_diff 1), step 0.05."""
     from libbicos_amd import scene
     L, R, gt = scene.generate(scene.C4_4096x3000_U16, seed=104, height=6)
