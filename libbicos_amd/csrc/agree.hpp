@@ -59,7 +59,7 @@ __device__ __forceinline__ TPrec nxcorr(const float (&p0)[NMAX], const float (&p
 }
 
 template<typename TIn, int NMAX, int N_EXACT>
-__device__ __forceinline__ void gather(float (&p)[NMAX], const FrameTable& f,
+__device__ __forceinline__ void gather(float (&p)[NMAX], const Stack& f,
                                        Depth<NMAX, N_EXACT> n, int row, int col) {
 #pragma unroll
     for (int t = 0; t < NMAX; ++t)
@@ -70,7 +70,7 @@ __device__ __forceinline__ void gather(float (&p)[NMAX], const FrameTable& f,
 // the corrmap is written (NaN where no correlation is evaluated).
 template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree(int16_t* __restrict__ disp, int64_t dstep,
-                                               const FrameTable f0, const FrameTable f1, int n_rt,
+                                               const Stack f0, const Stack f1, int n_rt,
                                                int W, float thr, int has_minvar, float minvar,
                                                TPrec* __restrict__ corrmap, int64_t cstep) {
     const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
@@ -98,13 +98,54 @@ __global__ __launch_bounds__(256) void k_agree(int16_t* __restrict__ disp, int64
         corrmap[(size_t)row * cstep + col] = nxc;
 }
 
+// The grid scan of agree_subpixel (agree.hpp:160-172): NCC of the left pixel
+// against the interpolated right samples at each x, first strict maximum.
+// WRAP = false drops the cast of the interpolated sample (caller proves it
+// is the identity).
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT, bool WRAP>
+__device__ __forceinline__ void subpixel_scan(const float (&a)[NMAX], const float (&b)[NMAX],
+                                              const float (&c)[NMAX], const TPrec (&d0)[NMAX],
+                                              TPrec var0, Depth<NMAX, N_EXACT> n, float step,
+                                              bool mv, float minvar, float& best_x, TPrec& best) {
+    for (float x = -1.0f; x <= 1.0f; x += step) {
+        float it[NMAX];
+        TPrec mean1 = 0;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t) {
+            it[t] = 0.0f;
+            if (n.has(t)) {
+                const float v = a[t] * x * x + b[t] * x + c[t];
+                if constexpr (WRAP)
+                    it[t] = (float)(TIn)(int)__builtin_rintf(v);
+                else
+                    it[t] = __builtin_rintf(v);
+                mean1 += (TPrec)it[t];
+            }
+        }
+        mean1 /= (TPrec)n.get();
+        TPrec covar = 0, var1 = 0;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t)
+            if (n.has(t)) {
+                const TPrec d1 = (TPrec)it[t] - mean1;
+                covar = pfma(d0[t], d1, covar);
+                var1 = pfma(d1, d1, var1);
+            }
+        const TPrec nxc = (mv && var1 < (TPrec)minvar) ? (TPrec)-1 : covar / psqrt(var0 * var1);
+        if (best < nxc) {
+            best_x = x;
+            best = nxc;
+        }
+    }
+}
+
 // agree_subpixel (agree.hpp:94-185). Writes every output pixel (NaN =
 // invalid) and every corrmap pixel. The left-side mean, centred samples and
 // variance do not depend on x and are computed once (same values as the
 // reference's per-x recomputation).
 template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree_subpixel(
-    const int16_t* __restrict__ disp, int64_t dstep, const FrameTable f0, const FrameTable f1,
+    const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1,
     int n_rt, int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
     int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep) {
     const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
@@ -159,37 +200,40 @@ __global__ __launch_bounds__(256) void k_agree_subpixel(
                 }
             }
 
+            // The cast (TIn)(int)rint(v) of agree.hpp:164 is the identity when
+            // rint(v) lies in [0, max(TIn)]. On x in [-1, 1] the parabola
+            // through (-1, y0), (0, y1), (1, y2) ranges over [min, max] of y0,
+            // y2 and, when its vertex x* = -b / 2a lies inside (|b| < 2|a|),
+            // the vertex value c - b^2 / 4a (magnitude below 2^15 there, so its
+            // float error is < 0.01). Requiring that range to stay within
+            // [1, max - 1] leaves margin for the < 0.05 float error of the
+            // evaluation, so every grid sample is in range and the cast can be
+            // skipped, bit-identically (and no -0.0 can arise). Waves where
+            // every pixel qualifies take the cast-free scan.
+            constexpr float VMAX = (float)(TIn)~(TIn)0;
+            bool in_range = true;
+#pragma unroll
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) {
+                    const float y0 = c[t] - b[t] + a[t], y2 = c[t] + b[t] + a[t];
+                    float lo = fminf(y0, y2), hi = fmaxf(y0, y2);
+                    if (__builtin_fabsf(b[t]) < 2.0f * __builtin_fabsf(a[t])) {
+                        const float v = c[t] - b[t] * b[t] / (4.0f * a[t]);
+                        lo = fminf(lo, v);
+                        hi = fmaxf(hi, v);
+                    }
+                    in_range = in_range && lo >= 1.0f && hi <= VMAX - 1.0f;
+                }
+
             float best_x = 0.0f;
             TPrec best = (TPrec)-1;
             if (!(mv && var0 < (TPrec)minvar)) {
-                for (float x = -1.0f; x <= 1.0f; x += step) {
-                    float it[NMAX];
-                    TPrec mean1 = 0;
-#pragma unroll
-                    for (int t = 0; t < NMAX; ++t) {
-                        it[t] = 0.0f;
-                        if (n.has(t)) {
-                            const float v = a[t] * x * x + b[t] * x + c[t];
-                            it[t] = (float)(TIn)(int)__builtin_rintf(v);
-                            mean1 += (TPrec)it[t];
-                        }
-                    }
-                    mean1 /= (TPrec)n.get();
-                    TPrec covar = 0, var1 = 0;
-#pragma unroll
-                    for (int t = 0; t < NMAX; ++t)
-                        if (n.has(t)) {
-                            const TPrec d1 = (TPrec)it[t] - mean1;
-                            covar = pfma(d0[t], d1, covar);
-                            var1 = pfma(d1, d1, var1);
-                        }
-                    const TPrec nxc = (mv && var1 < (TPrec)minvar) ? (TPrec)-1
-                                                                   : covar / psqrt(var0 * var1);
-                    if (best < nxc) {
-                        best_x = x;
-                        best = nxc;
-                    }
-                }
+                if (__all(in_range))
+                    subpixel_scan<TIn, TPrec, NMAX, N_EXACT, false>(a, b, c, d0, var0, n, step, mv,
+                                                                     minvar, best_x, best);
+                else
+                    subpixel_scan<TIn, TPrec, NMAX, N_EXACT, true>(a, b, c, d0, var0, n, step, mv,
+                                                                    minvar, best_x, best);
             }
             corr = best;
             if (!(best < (TPrec)thr))

@@ -21,6 +21,7 @@
 
 #include "../../include/bicos_hip.h"
 #include "agree.hpp"
+#include "agree_half.hpp"
 #include "search.hpp"
 #include "search_mfma.hpp"
 #include "transform.hpp"
@@ -193,36 +194,92 @@ int check_stack(const bicos_image* s, uint32_t n, StackInfo* info, FrameTable* f
     return BICOS_OK;
 }
 
+// Frames one (n, rows, cols) allocation would have: equal row steps and a
+// constant distance between consecutive frames.
+bool uniformly_spaced(const FrameTable& f, uint32_t n) {
+    const int64_t fs = n > 1 ? (int64_t)(f.ptr[1] - f.ptr[0]) : 0;
+    for (uint32_t t = 1; t < n; ++t)
+        if (f.step[t] != f.step[0] || (int64_t)(f.ptr[t] - f.ptr[0]) != (int64_t)t * fs) return false;
+    return true;
+}
+
+size_t packed_bytes(const FrameTable& f, uint32_t n, const StackInfo& i) {
+    return uniformly_spaced(f, n) ? 0 : (size_t)n * i.rows * i.cols * elem_size(i.depth);
+}
+
+// The strided view of a stack: in place, or packed into `scratch` (which
+// must hold packed_bytes()) by one gather launch.
+int make_stack(const FrameTable& f, uint32_t n, const StackInfo& i, void* scratch, Stack* out,
+               hipStream_t s);
+
 /* --------------------------------------------------------------- dispatch */
 
+// 4-pixel-per-lane transform kernels (else one pixel per lane)
+bool g_transform_vec = true;
+
+bool rows_aligned(const Stack& f, size_t align) {
+    return (uint64_t)f.base % align == 0 && (uint64_t)f.fstride % align == 0 &&
+           (uint64_t)f.step % align == 0;
+}
+
+// Transform of one or two views (f1/out1 null: one view). The 4-pixel
+// vectorised kernels need rows aligned to 4 samples; otherwise the
+// one-pixel-per-lane kernels run.
 template<typename TIn>
-int run_transform(const FrameTable& f, uint32_t n, int mode, int W, int H, int words,
-                  uint32_t* out, hipStream_t s) {
-    const dim3 grid((W + 255) / 256, H), block(256);
-    if (mode == BICOS_MODE_LIMITED) {
-        const int natural = words_for_bits(required_bits(n, mode));
-        switch (natural) {
-            case 1: LAUNCH("transform_limited_32", (k_transform_limited<TIn, 1, 9>), grid, block, 0, s, f, (int)n, W, words, out); break;
-            case 2: LAUNCH("transform_limited_64", (k_transform_limited<TIn, 2, 17>), grid, block, 0, s, f, (int)n, W, words, out); break;
-            case 4: LAUNCH("transform_limited_128", (k_transform_limited<TIn, 4, 33>), grid, block, 0, s, f, (int)n, W, words, out); break;
-            case 8: LAUNCH("transform_limited_256", (k_transform_limited<TIn, 8, 65>), grid, block, 0, s, f, (int)n, W, words, out); break;
-            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
-        }
-    } else {
-        switch (n) {
-#define TF(N) case N: LAUNCH("transform_full", (k_transform_full<TIn, N>), grid, block, 0, s, f, W, words, out); break;
-            TF(2) TF(3) TF(4) TF(5) TF(6) TF(7) TF(8) TF(9) TF(10) TF(11) TF(12) TF(13) TF(14) TF(15) TF(16)
+int run_transform(const Stack& f0, const Stack* f1, uint32_t n, int mode, int W, int H,
+                  int words, uint32_t* out0, uint32_t* out1, hipStream_t s) {
+    const int views = f1 ? 2 : 1;
+    const bool vec = g_transform_vec && rows_aligned(f0, 4 * sizeof(TIn)) &&
+                     (!f1 || rows_aligned(*f1, 4 * sizeof(TIn)));
+    const Stack& g1 = f1 ? *f1 : f0;
+    if (vec) {
+        const dim3 grid((W + 256 * TPX - 1) / (256 * TPX), H, views), block(256);
+        if (mode == BICOS_MODE_LIMITED) {
+            switch (words_for_bits(required_bits(n, mode))) {
+                case 1: LAUNCH("transform", (k_transform_limited_v4<TIn, 1, 9>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
+                case 2: LAUNCH("transform", (k_transform_limited_v4<TIn, 2, 17>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
+                case 4: LAUNCH("transform", (k_transform_limited_v4<TIn, 4, 33>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
+                case 8: LAUNCH("transform", (k_transform_limited_v4<TIn, 8, 65>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
+                default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+            }
+        } else {
+            switch (n) {
+#define TF(N) case N: LAUNCH("transform", (k_transform_full_v4<TIn, N>), grid, block, 0, s, f0, g1, W, words, out0, out1); break;
+                TF(2) TF(3) TF(4) TF(5) TF(6) TF(7) TF(8) TF(9) TF(10) TF(11) TF(12) TF(13) TF(14) TF(15) TF(16)
 #undef TF
-            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+                default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+            }
+        }
+        return BICOS_OK;
+    }
+    const dim3 grid((W + 255) / 256, H), block(256);
+    for (int v = 0; v < views; ++v) {
+        const Stack& f = v ? g1 : f0;
+        uint32_t* out = v ? out1 : out0;
+        if (mode == BICOS_MODE_LIMITED) {
+            switch (words_for_bits(required_bits(n, mode))) {
+                case 1: LAUNCH("transform_scalar", (k_transform_limited<TIn, 1, 9>), grid, block, 0, s, f, (int)n, W, words, out); break;
+                case 2: LAUNCH("transform_scalar", (k_transform_limited<TIn, 2, 17>), grid, block, 0, s, f, (int)n, W, words, out); break;
+                case 4: LAUNCH("transform_scalar", (k_transform_limited<TIn, 4, 33>), grid, block, 0, s, f, (int)n, W, words, out); break;
+                case 8: LAUNCH("transform_scalar", (k_transform_limited<TIn, 8, 65>), grid, block, 0, s, f, (int)n, W, words, out); break;
+                default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+            }
+        } else {
+            switch (n) {
+#define TF(N) case N: LAUNCH("transform_scalar", (k_transform_full<TIn, N>), grid, block, 0, s, f, W, words, out); break;
+                TF(2) TF(3) TF(4) TF(5) TF(6) TF(7) TF(8) TF(9) TF(10) TF(11) TF(12) TF(13) TF(14) TF(15) TF(16)
+#undef TF
+                default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
+            }
         }
     }
     return BICOS_OK;
 }
 
-int transform_any(const FrameTable& f, int depth, uint32_t n, int mode, int W, int H, int words,
-                  uint32_t* out, hipStream_t s) {
-    return depth == BICOS_8U ? run_transform<uint8_t>(f, n, mode, W, H, words, out, s)
-                             : run_transform<uint16_t>(f, n, mode, W, H, words, out, s);
+int transform_any(const Stack& f0, const Stack* f1, int depth, uint32_t n, int mode,
+                  int W, int H, int words, uint32_t* out0, uint32_t* out1, hipStream_t s) {
+    return depth == BICOS_8U ? run_transform<uint8_t>(f0, f1, n, mode, W, H, words, out0, out1, s)
+                             : run_transform<uint16_t>(f0, f1, n, mode, W, H, words, out0, out1, s);
 }
 
 constexpr size_t SEARCH_LDS_BYTES = 64 * 1024;
@@ -276,9 +333,12 @@ int run_search(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, 
                hipStream_t s) {
     if (g_search_impl == BICOS_SEARCH_MFMA) {
         switch (words) {
-            case 1: return run_search_mfma_w<1, 4, 8, 512>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 2: return run_search_mfma_w<2, 4, 8, 512>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 4: return run_search_mfma_w<4, 4, 8, 256>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            // 2 query tiles per wave, 8 waves per block, candidate chunks
+            // filling 64 KiB of LDS (measured best of the tried shapes for
+            // 1920-wide rows of 128-bit descriptors)
+            case 1: return run_search_mfma_w<1, 2, 8, 1344>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 2: return run_search_mfma_w<2, 2, 8, 800>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 4: return run_search_mfma_w<4, 2, 8, 448>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
             case 8: return run_search_mfma_w<8, 2, 8, 224>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
@@ -319,7 +379,7 @@ int with_nmax(uint32_t n, F&& f) {
 }
 
 template<typename TIn, typename TPrec>
-int run_agree_t(int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1,
+int run_agree_t(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                 uint32_t n, int W, int H, float thr, int has_mv, float mv, TPrec* corr,
                 int64_t cstep, hipStream_t s) {
     const dim3 grid((W + 255) / 256, H), block(256);
@@ -331,7 +391,7 @@ int run_agree_t(int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameT
 }
 
 template<typename TIn, typename TPrec>
-int run_subpixel_t(const int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1,
+int run_subpixel_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                    uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
                    float* out, int64_t ostep, TPrec* corr, int64_t cstep, hipStream_t s) {
     const dim3 grid((W + 255) / 256, H), block(256);
@@ -342,7 +402,9 @@ int run_subpixel_t(const int16_t* disp, int64_t dstep, const FrameTable& f0, con
     });
 }
 
-int run_agree(int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1, int depth,
+// HALF_MFMA concerns the subpixel candidates only; the single-point
+// agreement is the exact FP32 one.
+int run_agree(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
               uint32_t n, int W, int H, float thr, int has_mv, float mv_n, int precision,
               void* corr, int64_t cstep, hipStream_t s) {
     if (precision == BICOS_PRECISION_DOUBLE) {
@@ -355,10 +417,30 @@ int run_agree(int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTab
         : run_agree_t<uint16_t, float>(disp, dstep, f0, f1, n, W, H, thr, has_mv, mv_n, (float*)corr, cstep, s);
 }
 
-int run_subpixel(const int16_t* disp, int64_t dstep, const FrameTable& f0, const FrameTable& f1,
+template<typename TIn>
+int run_subpixel_half_t(const int16_t* disp, int64_t dstep, const Stack& f0,
+                        const Stack& f1, uint32_t n, int W, int H, float thr, float step,
+                        int has_mv, float mv, float* out, int64_t ostep, float* corr,
+                        int64_t cstep, hipStream_t s) {
+    const dim3 grid((W + 63) / 64, H), block(64);
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        constexpr int NMAX = decltype(NM)::value;
+        constexpr size_t lds = sizeof(HalfLds<(NMAX + 15) / 16 * 16>);
+        LAUNCH("agree_subpixel_half", (k_agree_subpixel_half<TIn, NMAX, decltype(NE)::value, 1>), grid, block, lds, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep);
+        return BICOS_OK;
+    });
+}
+
+int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                  int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
                  float mv_n, int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
                  hipStream_t s) {
+    if (precision == BICOS_PRECISION_HALF_MFMA) {
+        return depth == BICOS_8U
+            ? run_subpixel_half_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s)
+            : run_subpixel_half_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s);
+    }
     if (precision == BICOS_PRECISION_DOUBLE) {
         return depth == BICOS_8U
             ? run_subpixel_t<uint8_t, double>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, s)
@@ -404,9 +486,49 @@ int check_out(const bicos_image* img, int rows, int cols, int type, const char* 
     return BICOS_OK;
 }
 
+int make_stack(const FrameTable& f, uint32_t n, const StackInfo& i, void* scratch, Stack* out,
+               hipStream_t s) {
+    if (uniformly_spaced(f, n)) {
+        out->base = (const uint8_t*)f.ptr[0];
+        out->fstride = n > 1 ? (int64_t)(f.ptr[1] - f.ptr[0]) : 0;
+        out->step = f.step[0];
+        return BICOS_OK;
+    }
+    const dim3 grid((i.cols + 1023) / 1024, i.rows, n), block(256);
+    if (i.depth == BICOS_8U)
+        LAUNCH("pack_frames", k_pack_frames<uint8_t>, grid, block, 0, s, f, i.cols, (uint8_t*)scratch);
+    else
+        LAUNCH("pack_frames", k_pack_frames<uint16_t>, grid, block, 0, s, f, i.cols, (uint16_t*)scratch);
+    const int es = elem_size(i.depth);
+    out->base = (const uint8_t*)scratch;
+    out->fstride = (int64_t)i.rows * i.cols * es;
+    out->step = (int64_t)i.cols * es;
+    return BICOS_OK;
+}
+
 int current_device(int* dev) {
     HIP_TRY(hipGetDevice(dev));
     return BICOS_OK;
+}
+
+// Runs fn(stacks...) with strided views of up to two stacks, packing
+// non-uniformly spaced ones into the workspace.
+template<typename F>
+int with_stacks(const FrameTable* f, const StackInfo* info, int count, uint32_t n, hipStream_t s,
+                F&& fn) {
+    size_t pk[2] = {0, 0};
+    for (int k = 0; k < count; ++k) pk[k] = align_up(packed_bytes(f[k], n, info[k]), 256);
+    int dev, rc;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    void* ws = nullptr;
+    if ((rc = ws_acquire(dev, pk[0] + pk[1] + 256, s, &ws))) return rc;
+    Stack v[2];
+    for (int k = 0; k < count && !rc; ++k)
+        rc = make_stack(f[k], n, info[k], (uint8_t*)ws + (k ? pk[0] : 0), &v[k], s);
+    if (!rc) rc = fn(v);
+    int rc2 = ws_release(dev, s);
+    return rc ? rc : rc2;
 }
 
 }  // namespace
@@ -460,7 +582,8 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown transform mode");
     if (cfg->variant != BICOS_VARIANT_NO_DUPLICATES && cfg->variant != BICOS_VARIANT_CONSISTENCY)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented");  // cuda.cu.in:51
-    if (cfg->precision != BICOS_PRECISION_SINGLE && cfg->precision != BICOS_PRECISION_DOUBLE)
+    if (cfg->precision != BICOS_PRECISION_SINGLE && cfg->precision != BICOS_PRECISION_DOUBLE &&
+        cfg->precision != BICOS_PRECISION_HALF_MFMA)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
 
     const int bits = required_bits(n, cfg->mode);
@@ -482,22 +605,28 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     const size_t desc_bytes = align_up(npx * words * 4, 256);
     const size_t raw_bytes = subpix ? align_up(npx * 2, 256) : 0;
     const size_t rev_bytes = consistency ? align_up(npx * 2, 256) : 0;
+    const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
 
     int dev;
     if ((rc = current_device(&dev))) return rc;
     std::lock_guard<std::mutex> lk(g_ws_mu);
     void* ws;
-    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + rev_bytes, s, &ws))) return rc;
+    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + rev_bytes + pk0 + pk1, s, &ws))) return rc;
     uint8_t* p = (uint8_t*)ws;
     uint32_t* desc0 = (uint32_t*)p;
     uint32_t* desc1 = (uint32_t*)(p + desc_bytes);
     int16_t* raw = subpix ? (int16_t*)(p + 2 * desc_bytes) : nullptr;
     int16_t* rev = consistency ? (int16_t*)(p + 2 * desc_bytes + raw_bytes) : nullptr;
+    uint8_t* pack0 = p + 2 * desc_bytes + raw_bytes + rev_bytes;
+    uint8_t* pack1 = pack0 + pk0;
 
     auto body = [&]() -> int {
         int r;
-        if ((r = transform_any(f0, i0.depth, n, cfg->mode, W, H, words, desc0, s))) return r;
-        if ((r = transform_any(f1, i0.depth, n, cfg->mode, W, H, words, desc1, s))) return r;
+        Stack v0, v1;
+        if ((r = make_stack(f0, n, i0, pack0, &v0, s))) return r;
+        if ((r = make_stack(f1, n, i1, pack1, &v1, s))) return r;
+        if ((r = transform_any(v0, &v1, i0.depth, n, cfg->mode, W, H, words, desc0, desc1, s)))
+            return r;
 
         int flags = 1, lr = -1;  // cpu.cpp:70-77
         if (consistency) {
@@ -513,10 +642,10 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
         void* cm = (corrmap && thr) ? corrmap->data : nullptr;
         const int64_t cstep = cm ? corrmap->step / elem_size(corrmap->type) : 0;
         if (subpix)
-            return run_subpixel(raw, W, f0, f1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+            return run_subpixel(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                                 cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision,
                                 (float*)disparity->data, disparity->step / 4, cm, cstep, s);
-        return run_agree(d16, d16_step, f0, f1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+        return run_agree(d16, d16_step, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                          cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, s);
     };
     rc = body();
@@ -563,8 +692,11 @@ int bicos_hip_transform(const bicos_image* frames, uint32_t n, int32_t mode, int
     if (words < natural)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor too narrow for this stack");
     if (!out) return fail(BICOS_ERR_EXCEPTION, "null output");
-    return transform_any(f, info.depth, n, mode, info.cols, info.rows, words, out,
-                         (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    return with_stacks(&f, &info, 1, n, s, [&](const Stack* v) {
+        return transform_any(v[0], nullptr, info.depth, n, mode, info.cols, info.rows, words, out,
+                             nullptr, s);
+    });
 }
 
 int bicos_hip_search(const uint32_t* desc0, const uint32_t* desc1, int32_t rows, int32_t cols,
@@ -591,36 +723,44 @@ int bicos_hip_search(const uint32_t* desc0, const uint32_t* desc1, int32_t rows,
 int bicos_hip_agree(int16_t* disparity, const bicos_image* stack0, const bicos_image* stack1,
                     uint32_t n, float thr, int32_t has_min_variance, float min_variance,
                     int32_t precision, void* corrmap, void* stream) {
-    StackInfo i0, i1;
-    FrameTable f0, f1;
+    StackInfo i[2];
+    FrameTable f[2];
     int rc;
-    if ((rc = check_stack(stack0, n, &i0, &f0))) return rc;
-    if ((rc = check_stack(stack1, n, &i1, &f1))) return rc;
+    if ((rc = check_stack(stack0, n, &i[0], &f[0]))) return rc;
+    if ((rc = check_stack(stack1, n, &i[1], &f[1]))) return rc;
     if (!disparity) return fail(BICOS_ERR_EXCEPTION, "null disparity");
-    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE)
+    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE &&
+        precision != BICOS_PRECISION_HALF_MFMA)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
     const float mv_n = has_min_variance ? min_variance * (float)n : 0.0f;
-    return run_agree(disparity, i0.cols, f0, f1, i0.depth, n, i0.cols, i0.rows, thr,
-                     has_min_variance, mv_n, precision, corrmap, i0.cols, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    return with_stacks(f, i, 2, n, s, [&](const Stack* v) {
+        return run_agree(disparity, i[0].cols, v[0], v[1], i[0].depth, n, i[0].cols, i[0].rows,
+                         thr, has_min_variance, mv_n, precision, corrmap, i[0].cols, s);
+    });
 }
 
 int bicos_hip_agree_subpixel(const int16_t* disparity, const bicos_image* stack0,
                              const bicos_image* stack1, uint32_t n, float thr,
                              float subpixel_step, int32_t has_min_variance, float min_variance,
                              int32_t precision, float* out, void* corrmap, void* stream) {
-    StackInfo i0, i1;
-    FrameTable f0, f1;
+    StackInfo i[2];
+    FrameTable f[2];
     int rc;
-    if ((rc = check_stack(stack0, n, &i0, &f0))) return rc;
-    if ((rc = check_stack(stack1, n, &i1, &f1))) return rc;
+    if ((rc = check_stack(stack0, n, &i[0], &f[0]))) return rc;
+    if ((rc = check_stack(stack1, n, &i[1], &f[1]))) return rc;
     if (!disparity || !out) return fail(BICOS_ERR_EXCEPTION, "null buffer");
     if ((rc = check_step(subpixel_step))) return rc;
-    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE)
+    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE &&
+        precision != BICOS_PRECISION_HALF_MFMA)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
     const float mv_n = has_min_variance ? min_variance * (float)n : 0.0f;
-    return run_subpixel(disparity, i0.cols, f0, f1, i0.depth, n, i0.cols, i0.rows, thr,
-                        subpixel_step, has_min_variance, mv_n, precision, out, i0.cols, corrmap,
-                        i0.cols, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    return with_stacks(f, i, 2, n, s, [&](const Stack* v) {
+        return run_subpixel(disparity, i[0].cols, v[0], v[1], i[0].depth, n, i[0].cols, i[0].rows,
+                            thr, subpixel_step, has_min_variance, mv_n, precision, out,
+                            i[0].cols, corrmap, i[0].cols, s);
+    });
 }
 
 int bicos_hip_timing_enable(int32_t enable) {

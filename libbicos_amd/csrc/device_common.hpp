@@ -17,18 +17,40 @@ struct FrameTable {
     int64_t step[MAX_FRAMES];
 };
 
+// The frames of one view as the compute kernels see them: frame t, row r
+// starts at base + t * fstride + r * step (bytes). Uniformly spaced frames
+// (one (n, rows, cols) allocation) are used in place; others are packed
+// into this form first (k_pack_frames).
+struct Stack {
+    const uint8_t* base;
+    int64_t fstride;
+    int64_t step;
+};
+
 constexpr int16_t INVALID16 = -32768;  // INVALID_DISP<int16_t> (include/common.hpp:34-37)
 
 template<typename T>
-__device__ __forceinline__ uint32_t load_sample(const FrameTable& f, int t, int row, int col) {
-    const T* p = (const T*)(f.ptr[t] + (uint64_t)row * (uint64_t)f.step[t]);
-    return (uint32_t)__builtin_nontemporal_load(p + col);
+__device__ __forceinline__ const T* frame_row(const Stack& f, int t, int row) {
+    return (const T*)(f.base + (int64_t)t * f.fstride + (int64_t)row * f.step);
 }
 
 template<typename T>
-__device__ __forceinline__ uint32_t load_sample_cached(const FrameTable& f, int t, int row, int col) {
-    const T* p = (const T*)(f.ptr[t] + (uint64_t)row * (uint64_t)f.step[t]);
-    return (uint32_t)p[col];
+__device__ __forceinline__ uint32_t load_sample_cached(const Stack& f, int t, int row, int col) {
+    return (uint32_t)frame_row<T>(f, t, row)[col];
+}
+
+// gathers n frame planes given by a pointer table into one dense stack
+// (frame stride rows * cols, row step cols), 4 samples per lane
+template<typename T>
+__global__ __launch_bounds__(256) void k_pack_frames(const FrameTable f, int W, T* __restrict__ out) {
+    const int col = (blockIdx.x * 256 + threadIdx.x) * 4, row = blockIdx.y, t = blockIdx.z;
+    if (col >= W)
+        return;
+    const T* src = (const T*)(f.ptr[t] + (uint64_t)row * (uint64_t)f.step[t]);
+    T* dst = out + ((size_t)t * gridDim.y + row) * W;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (col + k < W) dst[col + k] = src[col + k];
 }
 
 __device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {

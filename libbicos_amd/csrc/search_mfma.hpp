@@ -99,15 +99,49 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
         }
         __syncthreads();
 
-        for (int mt = 0; mt < cnp / 32; ++mt) {
+        // full tiles: three VALU ops per output; the NB product chains are
+        // independent, so matrix-core work of one overlaps the VALU work of
+        // the previous
+        const int nfull = cn / 32;
+        for (int mt = 0; mt < nfull; ++mt) {
             v4i aop[KS];
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 const uint4 v = lds[(mt * 32 + nl) * STRIDE + 2 * s + g];
                 aop[s] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
             }
-            const int tbase = c0 + mt * 32;  // first candidate column of the tile
-            const bool partial = tbase + 32 > W;
+            v16i acc[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], v16i{0}, 0, 0, 0);
+#pragma unroll
+                for (int s = 1; s < KS; ++s)
+                    acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[s], qop[b][s], acc[b], 0, 0, 0);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    // candidate row within the tile: 8 (i / 4) + 4 g + i % 4; the
+                    // 4 g part and the tile base live in the running state offset
+                    const uint32_t key = ((uint32_t)acc[b][i] << 16) + ((256u << 16) + 8u * (i / 4) + (i % 4));
+                    if constexpr (NODUPES) s2[b] = umed3(s1[b], s2[b], key);
+                    s1[b] = min(s1[b], key);
+                }
+                s1[b] -= 32u;
+                if constexpr (NODUPES) s2[b] -= 32u;
+            }
+        }
+        // ragged last tile of the row: candidates at or beyond W are masked
+        if (nfull * 32 < cn) {
+            const int mt = nfull;
+            v4i aop[KS];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const uint4 v = lds[(mt * 32 + nl) * STRIDE + 2 * s + g];
+                aop[s] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+            }
+            const int tbase = c0 + mt * 32;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 v16i acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], v16i{0}, 0, 0, 0);
@@ -116,17 +150,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
                     acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[s], qop[b][s], acc, 0, 0, 0);
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    // candidate row within the tile: 8 (i / 4) + 4 g + i % 4; the
-                    // 4 g part and the tile base live in the running state offset
-                    const uint32_t ci = (256u << 16) + 8u * (i / 4) + (i % 4);
-                    uint32_t key = ((uint32_t)acc[i] << 16) + ci;
-                    if (partial && tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = NO_KEY;
+                    uint32_t key = ((uint32_t)acc[i] << 16) + ((256u << 16) + 8u * (i / 4) + (i % 4));
+                    if (tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = NO_KEY;
                     if constexpr (NODUPES) s2[b] = umed3(s1[b], s2[b], key);
                     s1[b] = min(s1[b], key);
                 }
-            }
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
                 s1[b] -= 32u;
                 if constexpr (NODUPES) s2[b] -= 32u;
             }

@@ -40,7 +40,7 @@ __device__ __forceinline__ void store_desc(uint32_t* __restrict__ out, const uin
 // (3t for t < 2, 4t - 2 after that, when the pair-sum bit joins); the four
 // closing bits go to 3(n-2) for n < 4 and to 4n - 10 otherwise.
 template<typename TIn, int WORDS, int NMAX>
-__global__ __launch_bounds__(256) void k_transform_limited(const FrameTable f, int n, int W,
+__global__ __launch_bounds__(256) void k_transform_limited(const Stack f, int n, int W,
                                                            int out_words,
                                                            uint32_t* __restrict__ out) {
     const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_transform_limited(const FrameTable f, i
 // Full mode (descriptor_transform.hpp:75-123, descriptor_transform.cuh:30-89)
 // for a compile-time stack depth N (2..16).
 template<typename TIn, int N>
-__global__ __launch_bounds__(256) void k_transform_full(const FrameTable f, int W, int out_words,
+__global__ __launch_bounds__(256) void k_transform_full(const Stack f, int W, int out_words,
                                                         uint32_t* __restrict__ out) {
     constexpr int BITS = N * N - 2 * N + 3;
     constexpr int WORDS = BITS <= 32 ? 1 : BITS <= 64 ? 2 : BITS <= 128 ? 4 : 8;
@@ -147,6 +147,208 @@ __global__ __launch_bounds__(256) void k_transform_full(const FrameTable f, int 
         }
 
     store_desc(out + ((size_t)row * W + col) * out_words, w, out_words);
+}
+
+// ---------------------------------------------------------------------------
+// Vectorised form: each lane transforms PX = 4 adjacent pixels, reading 4
+// samples per frame with one 8-byte (u16) or 4-byte (u8) load. Samples are
+// kept as floats (exact: integers below 2^24), so every comparison
+// `x < y` is the sign bit of x - y. Bits of one emission group are shifted in
+// with v_alignbit (w = (w << 1) | (d >> 31)), last emission first, so the
+// group lands in natural order and is OR-ed into its word at a compile-time
+// position. blockIdx.z selects the view (0: out0 from f0, 1: out1 from f1).
+// Requires 8-byte (u16) / 4-byte (u8) aligned frame rows; the host checks.
+
+constexpr int TPX = 4;
+
+template<typename TIn>
+__device__ __forceinline__ void load_px4(const Stack& f, int t, int row, int col, int W,
+                                         float (&v)[TPX]) {
+    const TIn* p = frame_row<TIn>(f, t, row) + col;
+    if (col + TPX <= W) {
+        if constexpr (sizeof(TIn) == 2) {
+            const uint2 u = *(const uint2*)p;
+            v[0] = (float)(u.x & 0xFFFFu); v[1] = (float)(u.x >> 16);
+            v[2] = (float)(u.y & 0xFFFFu); v[3] = (float)(u.y >> 16);
+        } else {
+            const uint32_t u = *(const uint32_t*)p;
+            v[0] = (float)(u & 0xFFu); v[1] = (float)((u >> 8) & 0xFFu);
+            v[2] = (float)((u >> 16) & 0xFFu); v[3] = (float)(u >> 24);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < TPX; ++k) v[k] = col + k < W ? (float)p[k] : 0.0f;
+    }
+}
+
+// shift the comparison x < y into the low end of g
+__device__ __forceinline__ uint32_t shin(uint32_t g, float x, float y) {
+    return __builtin_amdgcn_alignbit(g, __float_as_uint(x - y), 31);
+}
+
+template<int WORDS>
+__device__ __forceinline__ void put_group(uint32_t (&w)[WORDS], int pos, int len, uint32_t g) {
+    // pos, len compile-time after unrolling; a group may straddle two words
+    w[pos >> 5] |= g << (pos & 31);
+    if ((pos & 31) + len > 32 && (pos >> 5) + 1 < WORDS)
+        w[(pos >> 5) + 1] |= g >> (32 - (pos & 31));
+}
+
+template<int WORDS>
+__device__ __forceinline__ void store_px(uint32_t* __restrict__ out, int W, int row, int col,
+                                         const uint32_t (&w)[TPX][WORDS], int out_words) {
+#pragma unroll
+    for (int k = 0; k < TPX; ++k)
+        if (col + k < W)
+            store_desc(out + ((size_t)row * W + col + k) * out_words, w[k], out_words);
+}
+
+// limited mode, runtime n <= NMAX (descriptor_transform.hpp:31-73)
+template<typename TIn, int WORDS, int NMAX>
+__global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, const Stack f1,
+                                                              int n, int W, int out_words,
+                                                              uint32_t* __restrict__ out0,
+                                                              uint32_t* __restrict__ out1) {
+    const int col = (blockIdx.x * 256 + threadIdx.x) * TPX, row = blockIdx.y;
+    if (col >= W)
+        return;
+    const Stack& f = blockIdx.z ? f1 : f0;
+    uint32_t* __restrict__ out = blockIdx.z ? out1 : out0;
+
+    float p[NMAX][TPX];
+    float av[TPX] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < NMAX; ++t) {
+        if (t < n) {
+            load_px4<TIn>(f, t, row, col, W, p[t]);
+#pragma unroll
+            for (int k = 0; k < TPX; ++k) av[k] += p[t][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < TPX; ++k) p[t][k] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TPX; ++k) av[k] /= (float)n;
+
+    uint32_t w[TPX][WORDS];
+#pragma unroll
+    for (int k = 0; k < TPX; ++k)
+#pragma unroll
+        for (int j = 0; j < WORDS; ++j) w[k][j] = 0u;
+
+    float prev0[TPX], prev1[TPX];
+#pragma unroll
+    for (int t = 0; t < NMAX - 2; ++t) {
+        if (t < n - 2) {
+            const int pos = t < 2 ? 3 * t : 4 * t - 2;
+#pragma unroll
+            for (int k = 0; k < TPX; ++k) {
+                const float a = p[t][k], b = p[t + 1][k], c = p[t + 2][k];
+                const float cur = a + b;
+                uint32_t g = 0;
+                if (t >= 2) g = shin(g, (t & 1) ? prev1[k] : prev0[k], cur);
+                g = shin(g, a, av[k]);
+                g = shin(g, a, c);
+                g = shin(g, a, b);
+                put_group(w[k], pos, t >= 2 ? 4 : 3, g);
+                if (t & 1) prev1[k] = cur; else prev0[k] = cur;
+            }
+        }
+    }
+
+    // closing bits: a<b, a<av, b<av, S(n-4) < S(n-2) (sentinel -1 for n < 4)
+    const int P = n >= 4 ? 4 * n - 10 : 3 * (n - 2);
+    const int wi = P >> 5, sh = P & 31;
+#pragma unroll
+    for (int k = 0; k < TPX; ++k) {
+        float a = 0.0f, b = 0.0f;
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t) {
+            if (t == n - 2) a = p[t][k];
+            if (t == n - 1) b = p[t][k];
+        }
+        const float prevf = n < 4 ? -1.0f : (((n - 2) & 1) ? prev1[k] : prev0[k]);
+        uint32_t g = 0;
+        g = shin(g, prevf, a + b);
+        g = shin(g, b, av[k]);
+        g = shin(g, a, av[k]);
+        g = shin(g, a, b);
+#pragma unroll
+        for (int j = 0; j < WORDS; ++j) {
+            if (j == wi) w[k][j] |= g << sh;
+            if (j == wi + 1 && sh > 28) w[k][j] |= g >> (32 - sh);
+        }
+    }
+    store_px(out, W, row, col, w, out_words);
+}
+
+// full mode, compile-time N (descriptor_transform.hpp:75-123); the whole
+// emission sequence is shifted in and each word bit-reversed at the end.
+template<typename TIn, int N>
+__global__ __launch_bounds__(256) void k_transform_full_v4(const Stack f0, const Stack f1,
+                                                           int W, int out_words,
+                                                           uint32_t* __restrict__ out0,
+                                                           uint32_t* __restrict__ out1) {
+    constexpr int BITS = N * N - 2 * N + 3;
+    constexpr int WORDS = BITS <= 32 ? 1 : BITS <= 64 ? 2 : BITS <= 128 ? 4 : 8;
+    const int col = (blockIdx.x * 256 + threadIdx.x) * TPX, row = blockIdx.y;
+    if (col >= W)
+        return;
+    const Stack& f = blockIdx.z ? f1 : f0;
+    uint32_t* __restrict__ out = blockIdx.z ? out1 : out0;
+
+    float p[N][TPX];
+    float av[TPX] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        load_px4<TIn>(f, t, row, col, W, p[t]);
+#pragma unroll
+        for (int k = 0; k < TPX; ++k) av[k] += p[t][k];
+    }
+#pragma unroll
+    for (int k = 0; k < TPX; ++k) av[k] /= (float)N;
+
+    uint32_t w[TPX][WORDS];
+#pragma unroll
+    for (int k = 0; k < TPX; ++k) {
+        uint32_t acc[WORDS];
+#pragma unroll
+        for (int j = 0; j < WORDS; ++j) acc[j] = 0u;
+        int pos = 0;
+        auto emit = [&](float x, float y) {
+            acc[pos >> 5] = __builtin_amdgcn_alignbit(acc[pos >> 5], __float_as_uint(x - y), 31);
+            ++pos;
+        };
+        float s[N - 1];
+#pragma unroll
+        for (int t = 0; t < N - 2; ++t) {
+            emit(p[t][k], p[t + 1][k]);
+            emit(p[t][k], p[t + 2][k]);
+            emit(p[t][k], av[k]);
+            s[t] = p[t][k] + p[t + 1][k];
+        }
+        s[N - 2] = p[N - 2][k] + p[N - 1][k];
+        emit(p[N - 2][k], p[N - 1][k]);
+        emit(p[N - 2][k], av[k]);
+        emit(p[N - 1][k], av[k]);
+#pragma unroll
+        for (int t = 0; t < N - 1; ++t)
+#pragma unroll
+            for (int i = 0; i < N - 1; ++i) {
+                if (i == t || i == t - 1 || i == t + 1)
+                    continue;
+                emit(s[t], s[i]);
+            }
+        // word j received min(32, BITS - 32 j) bits, first one highest
+#pragma unroll
+        for (int j = 0; j < WORDS; ++j) {
+            const int m = BITS - 32 * j >= 32 ? 32 : (BITS - 32 * j > 0 ? BITS - 32 * j : 0);
+            uint32_t r = __builtin_bitreverse32(acc[j]);
+            w[k][j] = m == 0 ? 0u : (m == 32 ? r : r >> (32 - m));
+        }
+    }
+    store_px(out, W, row, col, w, out_words);
 }
 
 }  // namespace bicos_hip

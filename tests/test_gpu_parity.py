@@ -78,14 +78,19 @@ def test_transform_wider_than_needed(gpu, oracle):
         assert np.array_equal(got, oracle.transform(s, full=False, words=words))
 
 
-def test_transform_pitched_frames(gpu, oracle):
-    """frames with a row pitch larger than the width (GpuMat-style padding)"""
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_transform_pitched_and_unaligned_frames(gpu, oracle, dtype):
+    """frames with a row pitch larger than the width (GpuMat-style padding),
+    and frames starting one sample off alignment (the one-pixel-per-lane
+    kernels), in both modes"""
     rng = np.random.default_rng(4)
-    big = rand_stack(rng, 12, 40, 512, np.uint8)
-    s = big[:, :, :333]
-    d = dev(big)[:, :, :333]
-    got = host(gpu.transform(d, 1), np.uint32)
-    assert np.array_equal(got, oracle.transform(np.ascontiguousarray(s), full=True))
+    big = rand_stack(rng, 12, 40, 512, dtype)
+    for off in (0, 1):
+        s = np.ascontiguousarray(big[:, :, off:off + 333])
+        d = dev(big)[:, :, off:off + 333]
+        for full in (False, True):
+            got = host(gpu.transform(d, int(full)), np.uint32)
+            assert np.array_equal(got, oracle.transform(s, full=full)), (off, full)
 
 
 # --------------------------------------------------------------- search
@@ -358,3 +363,69 @@ def test_errors_on_device(gpu):
         gpu.match_device(s[:1], s[:1])
     with pytest.raises(ValueError, match="subpixel_step"):
         gpu.match_device(s[:4], s[:4], gpu.Config(subpixel_step=-0.1))
+
+
+def test_match_separately_allocated_frames(gpu, oracle):
+    """frames in separate allocations (as with one GpuMat per image) are
+    packed into a strided stack first; results are unchanged"""
+    import torch
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=109, height=24, width=700, frames=20)
+    cfg = gpu.Config(nxcorr_threshold=0.9, subpixel_step=0.25, min_variance=1.0)
+    fl = [dev(L[t]).clone() for t in range(L.shape[0])]
+    fr = [dev(R[t]).clone() for t in range(R.shape[0])][::-1][::-1]
+    d, cm = gpu.match_device(fl, fr, cfg, want_corrmap=True)
+    exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
+    assert same(host(d), exp) and same(host(cm), exp_cm)
+    # transform alone on a shuffled frame order (non-monotonic addresses)
+    perm = [3, 0, 7, 1] + list(range(8, 20)) + [2, 4, 5, 6]
+    got = host(gpu.transform([fl[i] for i in perm], 0), np.uint32)
+    assert np.array_equal(got, oracle.transform(L[perm], full=False))
+
+
+def precision_stats(got, got_cm, exp, exp_cm):
+    fg, fe = np.isfinite(got), np.isfinite(exp)
+    both = fg & fe
+    dd = np.abs(got[both] - exp[both])
+    cmb = np.isfinite(got_cm) & np.isfinite(exp_cm)
+    dn = np.abs(got_cm[cmb].astype(np.float64) - exp_cm[cmb])
+    flips = (fg != fe).sum() / max(1, (fg | fe).sum())
+    return dict(valid_both=int(both.sum()), d_max=float(dd.max()), d_mean=float(dd.mean()),
+                d_over_0p1=float((dd > 0.1 + 1e-4).mean()), ncc_max=float(dn.max()),
+                ncc_mean=float(dn.mean()), flip_frac=float(flips))
+
+
+def test_precision_half_mfma_vs_fp64_oracle(gpu, oracle):
+    """HALF_MFMA subpixel mode on the 1920-wide 33-frame u16 workload: the
+    error of each result against the FP64 oracle (Precision::DOUBLE
+    semantics, agree.cuh:35-65) is measured and bounded."""
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=110, height=96)
+    cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
+                     precision=gpu.Precision.HalfMFMA)
+    got, got_cm = run_match(gpu, L, R, cfg)
+    exp, exp_cm = oracle.match(L, R, thr=0.96, step=0.1, min_variance=2.0, double=True,
+                               corrmap=True)
+    st = precision_stats(got, got_cm, exp, exp_cm)
+    print("HALF_MFMA vs FP64:", st)
+    assert st["valid_both"] > 0.5 * got.size
+    assert st["d_mean"] < 0.01 and st["d_over_0p1"] < 0.01 and st["d_max"] <= 1.0
+    assert st["ncc_mean"] < 1e-3
+    assert st["flip_frac"] < 0.005
+    # integer stages are shared with the exact path
+    plain, _ = run_match(gpu, L, R, gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1,
+                                               min_variance=2.0))
+    assert (np.isfinite(plain) | ~np.isfinite(got)).mean() > 0.99
+
+
+def test_precision_half_mfma_u8_full_mode(gpu, oracle):
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.C0_640x480_U8, seed=111, height=64)
+    cfg = gpu.Config(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full,
+                     precision=gpu.Precision.HalfMFMA)
+    got, got_cm = run_match(gpu, L, R, cfg)
+    exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.25, full=True, double=True, corrmap=True)
+    st = precision_stats(got, got_cm, exp, exp_cm)
+    print("HALF_MFMA u8 vs FP64:", st)
+    assert st["valid_both"] > 0.5 * got.size
+    assert st["d_mean"] < 0.05 and st["flip_frac"] < 0.02
