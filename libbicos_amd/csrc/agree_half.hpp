@@ -222,19 +222,34 @@ __global__ __launch_bounds__(WAVES * 64) void k_agree_subpixel_half(
         const float ac = 0.5f * (M01 - 2.0f * M11 + M12);
         const float bc = 0.5f * (M12 - M01);
         const float cc = M11;
-        float best_x = 0.0f, best = -1.0f;
+        // approximate scan: best and runner-up grid points
+        float best_x = 0.0f, best = -1.0f, second_x = 0.0f, second = -1.0f;
         if (!(mv && var0 < minvar)) {
             for (float x = -1.0f; x <= 1.0f; x += step) {
                 const float cov = (ua * x + ub) * x + uc;
                 const float v1 = (((aa * x + 2.0f * ab) * x + (bb + 2.0f * ac)) * x + 2.0f * bc) * x + cc;
                 const float nxc = (mv && v1 < minvar) ? -1.0f : cov * __builtin_amdgcn_rsqf(var0 * v1);
                 if (best < nxc) {
+                    second = best;
+                    second_x = best_x;
                     best = nxc;
                     best_x = x;
+                } else if (second < nxc) {
+                    second = nxc;
+                    second_x = x;
                 }
             }
         }
+        // winners rescored exactly; the runner-up too when the approximate
+        // scores are too close to rank (first strict maximum in grid order)
         corr = best > -1.0f ? rescore<TIn>(p0, f1, row, col1, best_x, n, mv, minvar) : -1.0f;
+        if (second > -1.0f && best - second < 2e-3f) {
+            const float c2 = rescore<TIn>(p0, f1, row, col1, second_x, n, mv, minvar);
+            if (c2 > corr || (c2 == corr && second_x < best_x)) {
+                corr = c2;
+                best_x = second_x;
+            }
+        }
         if (!(corr < thr)) res = (float)d - best_x;
     }
     if (col < W) {

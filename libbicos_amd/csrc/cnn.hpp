@@ -1,0 +1,351 @@
+// Learned disparity window: a small siamese CNN, its row correlation and
+// its training, in hand-written HIP (no reference counterpart).
+//
+// Network (shared weights for both views), on a 1/4-resolution grid:
+//   x0[t]  = (4x4 box mean of frame t) - (mean over the 4 frames), times in_scale
+//   a1 = relu(conv3x3(x0, 4 -> 32)), a2 = relu(conv3x3(a1)), a3 = relu(conv3x3(a2)),
+//   f  = conv3x3(a3)                       (32 features per cell)
+//   C[k](Y, X) = <fL(Y, X), fR(Y, X - (D0 + k))> / sqrt(32),  k < K
+// The window of a pixel is centred on 4 * (D0 + k*) with k* the refined
+// correlation peak of its cell. Training minimises the cross-entropy between
+// softmax_k C and the plain BICOS disparity of the cell (split linearly over
+// the two neighbouring bins). Weights are stored in BF16 for the forward
+// pass, accumulation is FP32; FP32 master weights and Adam moments live in
+// the caller's parameter buffers. Tensors are channel-planar [c][Y][X].
+#pragma once
+
+#include <hip/hip_bf16.h>
+
+#include "device_common.hpp"
+
+namespace bicos_hip {
+
+constexpr int CNN_C = 32;        // channels
+constexpr int CNN_IN = 4;        // input frames per view
+constexpr int CNN_DS = 4;        // downsampling factor of the grid
+constexpr int CNN_K = 80;        // correlation bins
+constexpr int CNN_D0 = -8;       // disparity (grid cells) of bin 0
+
+// flat parameter layout: w1 b1 w2 b2 w3 b3 w4 b4, w[co][ci][3][3]
+constexpr int CNN_W1 = CNN_C * CNN_IN * 9;
+constexpr int CNN_WL = CNN_C * CNN_C * 9;
+__host__ __device__ constexpr int cnn_w_off(int l) {
+    return l == 0 ? 0 : CNN_W1 + CNN_C + (l - 1) * (CNN_WL + CNN_C);
+}
+__host__ __device__ constexpr int cnn_b_off(int l) { return cnn_w_off(l) + (l == 0 ? CNN_W1 : CNN_WL); }
+constexpr int CNN_PARAMS = cnn_w_off(4);
+
+__device__ __forceinline__ float bf16_round(float x) {
+    return __bfloat162float(__float2bfloat16(x));
+}
+
+// x0 from the first CNN_IN frames of one view (strided stack)
+template<typename TIn>
+__global__ __launch_bounds__(256) void k_cnn_input(const Stack f, int W, int Hq, int Wq,
+                                                   float in_scale, float* __restrict__ x0) {
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    float q[CNN_IN], m = 0.0f;
+#pragma unroll
+    for (int t = 0; t < CNN_IN; ++t) {
+        float s = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < CNN_DS; ++dy) {
+            const TIn* r = frame_row<TIn>(f, t, Y * CNN_DS + dy) + X * CNN_DS;
+#pragma unroll
+            for (int dx = 0; dx < CNN_DS; ++dx) s += (float)r[dx];
+        }
+        q[t] = s * (1.0f / (CNN_DS * CNN_DS));
+        m += q[t];
+    }
+    m *= 1.0f / CNN_IN;
+    const size_t plane = (size_t)Hq * Wq;
+#pragma unroll
+    for (int t = 0; t < CNN_IN; ++t) x0[t * plane + (size_t)Y * Wq + X] = (q[t] - m) * in_scale;
+}
+
+// BF16 copy of the weights (biases stay FP32)
+__global__ void k_cnn_bf16_weights(const float* __restrict__ p, float* __restrict__ q) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= CNN_PARAMS) return;
+    bool is_bias = false;
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+        if (i >= cnn_b_off(l) && i < cnn_b_off(l) + CNN_C) is_bias = true;
+    q[i] = is_bias ? p[i] : bf16_round(p[i]);
+}
+
+// 3x3 convolution, zero padding; one lane per output cell computes all 32
+// output channels. Weights (already BF16-valued) staged in LDS.
+template<int CIN, bool RELU>
+__global__ __launch_bounds__(256) void k_cnn_conv(const float* __restrict__ x, const float* __restrict__ w,
+                                                  const float* __restrict__ b, int Hq, int Wq,
+                                                  float* __restrict__ y) {
+    __shared__ float ws[CNN_C * CIN * 9];
+    for (int i = threadIdx.x; i < CNN_C * CIN * 9; i += 256) ws[i] = w[i];
+    __syncthreads();
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const size_t plane = (size_t)Hq * Wq;
+    float acc[CNN_C];
+#pragma unroll
+    for (int co = 0; co < CNN_C; ++co) acc[co] = b[co];
+    for (int ci = 0; ci < CIN; ++ci) {
+        float v[9];
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int yy = Y + ky - 1, xx = X + kx - 1;
+                v[ky * 3 + kx] = (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq)
+                                     ? x[ci * plane + (size_t)yy * Wq + xx] : 0.0f;
+            }
+#pragma unroll
+        for (int co = 0; co < CNN_C; ++co)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[co] = __builtin_fmaf(ws[(co * CIN + ci) * 9 + k], v[k], acc[co]);
+    }
+#pragma unroll
+    for (int co = 0; co < CNN_C; ++co)
+        y[co * plane + (size_t)Y * Wq + X] = RELU ? fmaxf(acc[co], 0.0f) : acc[co];
+}
+
+// correlation volume C[k][Y][X] (masked bins: -1e30)
+__global__ __launch_bounds__(256) void k_cnn_corr(const float* __restrict__ fl, const float* __restrict__ fr,
+                                                  int Hq, int Wq, float* __restrict__ C) {
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const size_t plane = (size_t)Hq * Wq, at = (size_t)Y * Wq + X;
+    float l[CNN_C];
+#pragma unroll
+    for (int c = 0; c < CNN_C; ++c) l[c] = fl[c * plane + at];
+    for (int k = 0; k < CNN_K; ++k) {
+        const int xr = X - (CNN_D0 + k);
+        float s = -1e30f;
+        if (xr >= 0 && xr < Wq) {
+            s = 0.0f;
+#pragma unroll
+            for (int c = 0; c < CNN_C; ++c) s = __builtin_fmaf(l[c], fr[c * plane + (size_t)Y * Wq + xr], s);
+            s *= 0.17677669529663687f;  // 1 / sqrt(32)
+        }
+        C[k * plane + at] = s;
+    }
+}
+
+// softmax cross-entropy against the label disparity (full-resolution px,
+// NaN = unlabelled); writes dC (scaled by 1/count later) and the cell loss
+__global__ __launch_bounds__(256) void k_cnn_softmax_ce(const float* __restrict__ C,
+                                                        const float* __restrict__ label, int Hq,
+                                                        int Wq, float* __restrict__ dC,
+                                                        float* __restrict__ cell_loss) {
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const size_t plane = (size_t)Hq * Wq, at = (size_t)Y * Wq + X;
+    const float lab = label[at];
+    float mx = -1e30f;
+    for (int k = 0; k < CNN_K; ++k) mx = fmaxf(mx, C[k * plane + at]);
+    float z = 0.0f;
+    for (int k = 0; k < CNN_K; ++k) {
+        const float c = C[k * plane + at];
+        z += c > -1e29f ? __expf(c - mx) : 0.0f;
+    }
+    const float lz = mx + __logf(z);
+    float loss = 0.0f;
+    const bool has = !__builtin_isnan(lab);
+    float pos = has ? lab / CNN_DS - CNN_D0 : 0.0f;   // fractional bin
+    const bool in = has && pos >= 0.0f && pos <= (float)(CNN_K - 1);
+    const int k0 = in ? (int)floorf(pos) : -1;
+    const float w1 = in ? pos - k0 : 0.0f;
+    for (int k = 0; k < CNN_K; ++k) {
+        const float c = C[k * plane + at];
+        const float p = c > -1e29f ? __expf(c - lz) : 0.0f;
+        const float tgt = !in ? 0.0f : (k == k0 ? 1.0f - w1 : (k == k0 + 1 ? w1 : 0.0f));
+        if (in && tgt > 0.0f) loss -= tgt * (c - lz);
+        dC[k * plane + at] = in ? p - tgt : 0.0f;
+    }
+    cell_loss[at] = in ? loss : 0.0f;
+    cell_loss[plane + at] = in ? 1.0f : 0.0f;
+}
+
+// gradients of the correlation w.r.t. both feature maps
+__global__ __launch_bounds__(256) void k_cnn_corr_bwd(const float* __restrict__ fl, const float* __restrict__ fr,
+                                                      const float* __restrict__ dC, float scale, int Hq,
+                                                      int Wq, float* __restrict__ dfl,
+                                                      float* __restrict__ dfr) {
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const size_t plane = (size_t)Hq * Wq, at = (size_t)Y * Wq + X;
+    const float s = scale * 0.17677669529663687f;
+    float gl[CNN_C], gr[CNN_C];
+#pragma unroll
+    for (int c = 0; c < CNN_C; ++c) gl[c] = gr[c] = 0.0f;
+    for (int k = 0; k < CNN_K; ++k) {
+        const int d = CNN_D0 + k;
+        const int xr = X - d;            // this cell as the left one
+        if (xr >= 0 && xr < Wq) {
+            const float g = dC[k * plane + at] * s;
+            if (g != 0.0f)
+#pragma unroll
+                for (int c = 0; c < CNN_C; ++c) gl[c] = __builtin_fmaf(g, fr[c * plane + (size_t)Y * Wq + xr], gl[c]);
+        }
+        const int xl = X + d;            // this cell as the right one
+        if (xl >= 0 && xl < Wq) {
+            const float g = dC[k * plane + (size_t)Y * Wq + xl] * s;
+            if (g != 0.0f)
+#pragma unroll
+                for (int c = 0; c < CNN_C; ++c) gr[c] = __builtin_fmaf(g, fl[c * plane + (size_t)Y * Wq + xl], gr[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CNN_C; ++c) {
+        dfl[c * plane + at] = gl[c];
+        dfr[c * plane + at] = gr[c];
+    }
+}
+
+// dy_pre = dy * (y > 0) for layers followed by a ReLU (in place on dy)
+__global__ void k_cnn_relu_bwd(const float* __restrict__ y, float* __restrict__ dy, size_t count) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count && !(y[i] > 0.0f)) dy[i] = 0.0f;
+}
+
+// dx[ci] = sum_co sum_k w[co][ci][k] * dy[co] at the mirrored offset
+template<int CIN>
+__global__ __launch_bounds__(256) void k_cnn_conv_bwd_data(const float* __restrict__ dy, const float* __restrict__ w,
+                                                           int Hq, int Wq, float* __restrict__ dx) {
+    __shared__ float ws[CNN_C * CIN * 9];
+    for (int i = threadIdx.x; i < CNN_C * CIN * 9; i += 256) ws[i] = w[i];
+    __syncthreads();
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const size_t plane = (size_t)Hq * Wq;
+    float acc[CIN];
+#pragma unroll
+    for (int ci = 0; ci < CIN; ++ci) acc[ci] = 0.0f;
+    for (int co = 0; co < CNN_C; ++co) {
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                // y(Y', X') used x(Y' + ky - 1, X' + kx - 1): here Y' = Y - ky + 1
+                const int yy = Y - ky + 1, xx = X - kx + 1;
+                if (yy < 0 || yy >= Hq || xx < 0 || xx >= Wq) continue;
+                const float g = dy[co * plane + (size_t)yy * Wq + xx];
+#pragma unroll
+                for (int ci = 0; ci < CIN; ++ci) acc[ci] = __builtin_fmaf(ws[(co * CIN + ci) * 9 + ky * 3 + kx], g, acc[ci]);
+            }
+    }
+#pragma unroll
+    for (int ci = 0; ci < CIN; ++ci) dx[ci * plane + (size_t)Y * Wq + X] = acc[ci];
+}
+
+// per-block partial weight/bias gradients: one block per (co, row band);
+// lane = tap-input pair, summed over the band's cells (deterministic order)
+template<int CIN>
+__global__ __launch_bounds__(256) void k_cnn_conv_bwd_weight(const float* __restrict__ x, const float* __restrict__ dy,
+                                                             int Hq, int Wq, int rows_per_band,
+                                                             float* __restrict__ partial) {
+    constexpr int NT = CIN * 9 + 1;   // weights of one output channel + its bias
+    const int co = blockIdx.x, band = blockIdx.y;
+    const int y0 = band * rows_per_band, y1 = min(Hq, y0 + rows_per_band);
+    const size_t plane = (size_t)Hq * Wq;
+    for (int j = threadIdx.x; j < NT; j += 256) {
+        float s = 0.0f;
+        if (j == NT - 1) {
+            for (int Y = y0; Y < y1; ++Y)
+                for (int X = 0; X < Wq; ++X) s += dy[co * plane + (size_t)Y * Wq + X];
+        } else {
+            const int ci = j / 9, ky = (j % 9) / 3, kx = j % 3;
+            for (int Y = y0; Y < y1; ++Y) {
+                const int yy = Y + ky - 1;
+                if (yy < 0 || yy >= Hq) continue;
+                for (int X = 0; X < Wq; ++X) {
+                    const int xx = X + kx - 1;
+                    if (xx < 0 || xx >= Wq) continue;
+                    s = __builtin_fmaf(dy[co * plane + (size_t)Y * Wq + X], x[ci * plane + (size_t)yy * Wq + xx], s);
+                }
+            }
+        }
+        partial[((size_t)band * CNN_C + co) * NT + j] = s;
+    }
+}
+
+// sums band partials into the flat gradient (accumulating if acc != 0)
+template<int CIN>
+__global__ void k_cnn_reduce_weight(const float* __restrict__ partial, int bands, int layer,
+                                    int acc, float* __restrict__ grad) {
+    constexpr int NT = CIN * 9 + 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= CNN_C * NT) return;
+    const int co = i / NT, j = i % NT;
+    float s = 0.0f;
+    for (int b = 0; b < bands; ++b) s += partial[((size_t)b * CNN_C + co) * NT + j];
+    const int dst = j == NT - 1 ? cnn_b_off(layer) + co : cnn_w_off(layer) + co * CIN * 9 + j;
+    grad[dst] = acc ? grad[dst] + s : s;
+}
+
+__global__ void k_cnn_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                           const float* __restrict__ g, float lr, float b1, float b2, float eps,
+                           float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= CNN_PARAMS) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// window of every full-resolution pixel from its cell's correlation peak:
+// centre = 4 (D0 + k*) refined by a parabola through the peak bins,
+// half-width from the peak's softmax probability (hw_hi below p_conf)
+__global__ __launch_bounds__(256) void k_cnn_window(const float* __restrict__ C, int Hq, int Wq,
+                                                    int H, int W, float p_conf, int hw_lo,
+                                                    int hw_hi, float* __restrict__ center,
+                                                    int16_t* __restrict__ halfwidth) {
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const size_t plane = (size_t)Hq * Wq, at = (size_t)Y * Wq + X;
+    int kb = 0;
+    float mx = -1e30f;
+    for (int k = 0; k < CNN_K; ++k) {
+        const float c = C[k * plane + at];
+        if (c > mx) { mx = c; kb = k; }
+    }
+    float z = 0.0f;
+    for (int k = 0; k < CNN_K; ++k) {
+        const float c = C[k * plane + at];
+        z += c > -1e29f ? __expf(c - mx) : 0.0f;
+    }
+    const float pk = 1.0f / z;
+    float kf = (float)kb;
+    if (kb > 0 && kb < CNN_K - 1) {
+        const float cm = C[(kb - 1) * plane + at], cp = C[(kb + 1) * plane + at];
+        if (cm > -1e29f && cp > -1e29f) {
+            const float den = cm - 2.0f * mx + cp;
+            if (den < 0.0f) kf += fminf(fmaxf(0.5f * (cm - cp) / den, -0.5f), 0.5f);
+        }
+    }
+    const float c = CNN_DS * (CNN_D0 + kf);
+    const int16_t hw = (int16_t)(pk >= p_conf ? hw_lo : hw_hi);
+    for (int dy = 0; dy < CNN_DS; ++dy) {
+        const int y = Y * CNN_DS + dy;
+        if (y >= H) continue;
+        for (int dx = 0; dx < CNN_DS; ++dx) {
+            const int x = X * CNN_DS + dx;
+            if (x >= W) continue;
+            center[(size_t)y * W + x] = c;
+            halfwidth[(size_t)y * W + x] = hw;
+        }
+    }
+}
+
+}  // namespace bicos_hip
