@@ -157,9 +157,12 @@ __global__ __launch_bounds__(256) void k_cnn_softmax_ce(const float* __restrict_
     float loss = 0.0f;
     const bool has = !__builtin_isnan(lab);
     float pos = has ? lab / CNN_DS - CNN_D0 : 0.0f;   // fractional bin
-    const bool in = has && pos >= 0.0f && pos <= (float)(CNN_K - 1);
+    bool in = has && pos >= 0.0f && pos <= (float)(CNN_K - 1);
     const int k0 = in ? (int)floorf(pos) : -1;
     const float w1 = in ? pos - k0 : 0.0f;
+    // a cell whose target bins fall outside the right image is unlabelled
+    const int xr0 = X - (CNN_D0 + k0);
+    in = in && xr0 >= 0 && xr0 < Wq && (w1 == 0.0f || xr0 - 1 >= 0);
     for (int k = 0; k < CNN_K; ++k) {
         const float c = C[k * plane + at];
         const float p = c > -1e29f ? __expf(c - lz) : 0.0f;
