@@ -168,6 +168,88 @@ int bicos_hip_agree_subpixel(const int16_t* disparity, const bicos_image* stack0
                              float subpixel_step, int32_t has_min_variance, float min_variance,
                              int32_t precision, float* out, void* corrmap, void* stream);
 
+/* ---- learned search window (no reference counterpart) ----
+ *
+ * A siamese CNN on a 1/4-resolution grid of 4 x 4 cells (input: the 4x4 box
+ * means of the first 4 frames of each view minus their mean over those
+ * frames, times in_scale; 4 conv 3x3 layers of 32 channels, ReLU between,
+ * BF16 weights, FP32 accumulation) correlates left and right features over
+ * `bins` horizontal offsets and predicts per cell the centre and half-width
+ * of the disparity window the Hamming search then scans. Parameters: a flat
+ * device float array of bicos_hip_window_param_count() values, layout
+ * w1 b1 w2 b2 w3 b3 w4 b4 with w[out][in][3][3]. */
+
+typedef struct bicos_window_options {
+    float in_scale;        /* input scale; 0 = 8 / (largest sample value) */
+    float p_conf;          /* peak softmax probability for the narrow window */
+    int32_t hw_lo;         /* window half-width (px) of confident cells */
+    int32_t hw_hi;         /* ... and of the others */
+    int32_t bins;          /* correlation offsets: disparities 4 * (-8 .. bins - 9) px */
+    int32_t fallback;      /* 1: full-row search for pixels whose windowed match fails */
+} bicos_window_options;
+
+typedef struct bicos_window_stats {
+    int64_t pixels;            /* pixels matched */
+    int64_t fallback_pixels;   /* pixels sent to the full-row search */
+} bicos_window_stats;
+
+int bicos_hip_window_param_count(void);
+void bicos_hip_window_options_default(bicos_window_options* opts);
+
+/* Deterministic He-normal initial parameters (biases 0), host memory. */
+int bicos_hip_window_init(float* host_params, uint64_t seed);
+
+/* Training labels from a disparity map of the plain path (device image,
+ * BICOS_32F with NaN or BICOS_16S with -32768 = invalid): per cell the mean
+ * of its valid disparities when at least min_valid of its 16 pixels are
+ * valid, else NaN. label: device float, (rows/4) x (cols/4). */
+int bicos_hip_window_label(const bicos_image* disparity, int32_t min_valid, float* label,
+                           void* stream);
+
+/* Forward pass, softmax cross-entropy against the labels (a label splits
+ * linearly over its two neighbouring bins) and backward pass for one stack
+ * pair: ADDS the gradient of the summed loss to grad (device, param_count)
+ * and the loss sum and labelled-cell count to loss[0], loss[1] (device). */
+int bicos_hip_window_loss_grad(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                               const float* params, const bicos_window_options* opts,
+                               const float* label, float* grad, float* loss, void* stream);
+
+/* One Adam step on params with moments m, v (device, param_count each):
+ * the gradient used is grad / max(count[0], 1) (count: device, e.g. loss + 1);
+ * step counts from 1. */
+int bicos_hip_window_adam(float* params, float* m, float* v, const float* grad,
+                          const float* count, float lr, float beta1, float beta2, float eps,
+                          int32_t step, void* stream);
+
+/* Window of every cell: centre (disparity, px) and half-width (px); peak
+ * (nullable) receives the peak softmax probability. Device arrays of
+ * (rows/4) x (cols/4). */
+int bicos_hip_window_predict(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                             const float* params, const bicos_window_options* opts, float* center,
+                             int16_t* halfwidth, float* peak, void* stream);
+
+/* BICOS::match with the Hamming search limited to the predicted windows:
+ * same arguments, config semantics and outputs as bicos_hip_match, plus the
+ * parameters and options. Pixels whose windowed match the NCC stage rejects
+ * (and whose left variance does not reject every match) are searched over the
+ * full row and their NCC stage is run again (exact FP32 for SINGLE and
+ * HALF_MFMA, FP64 for DOUBLE). stats: NULL, or host memory filled after the
+ * call completes (the call then waits for the stream). Images need at least
+ * 4 rows and columns and fewer than 65536 rows. */
+int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                             const bicos_config* cfg, const float* params,
+                             const bicos_window_options* opts, bicos_image* disparity,
+                             bicos_image* corrmap, bicos_window_stats* stats, void* stream);
+
+/* ---- synthetic scenes (training and test inputs; the host generator's
+ *      construction evaluated on the device) ----
+ * depth 1 -> uint8, 2 -> uint16 samples; left/right: device n*H*W samples,
+ * frame-planar; disp: device H*W float ground truth (nullable). */
+int bicos_hip_scene_generate(int32_t width, int32_t height, int32_t n, int32_t depth, double dmin,
+                             double dmax, uint64_t seed, double lo, double hi, double sigma,
+                             double p_dark, double dark_value, void* left, void* right,
+                             float* disp, void* stream);
+
 /* ---- measurement ---- */
 
 /* Per-kernel GPU time accounting with HIP events recorded around every

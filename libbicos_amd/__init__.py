@@ -47,11 +47,29 @@ class _Config(C.Structure):
                 ("max_lr_diff", C.c_int32), ("no_dupes", C.c_int32)]
 
 
+class _WindowOptions(C.Structure):
+    _fields_ = [("in_scale", C.c_float), ("p_conf", C.c_float), ("hw_lo", C.c_int32),
+                ("hw_hi", C.c_int32), ("bins", C.c_int32), ("fallback", C.c_int32)]
+
+
+class _WindowStats(C.Structure):
+    _fields_ = [("pixels", C.c_int64), ("fallback_pixels", C.c_int64)]
+
+
 class _KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_int64), ("total_ms", C.c_double)]
 
 
 def _load():
+    # PyTorch-ROCm bundles its own HIP runtime under the same soname
+    # (libamdhip64.so.7). Whichever copy a process loads first is the one
+    # both use; loading /opt/rocm's copy first leaves torch's device
+    # initialisation and ours on different runtimes, and the device is not
+    # found. So torch, when present, is imported before the library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m libbicos_amd.build` "
@@ -78,6 +96,21 @@ def _load():
     L.bicos_hip_free.argtypes = [vp]
     L.bicos_hip_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
     L.bicos_hip_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
+    L.bicos_hip_window_options_default.argtypes = [P(_WindowOptions)]
+    L.bicos_hip_window_options_default.restype = None
+    L.bicos_hip_window_init.argtypes = [vp, C.c_uint64]
+    L.bicos_hip_window_label.argtypes = [P(_Image), i32, vp, vp]
+    L.bicos_hip_window_loss_grad.argtypes = [P(_Image), P(_Image), u32, vp, P(_WindowOptions), vp,
+                                             vp, vp, vp]
+    L.bicos_hip_window_adam.argtypes = [vp, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
+    L.bicos_hip_window_predict.argtypes = [P(_Image), P(_Image), u32, vp, P(_WindowOptions), vp,
+                                           vp, vp, vp]
+    L.bicos_hip_match_windowed.argtypes = [P(_Image), P(_Image), u32, P(_Config), vp,
+                                           P(_WindowOptions), P(_Image), P(_Image),
+                                           P(_WindowStats), vp]
+    d = C.c_double
+    L.bicos_hip_scene_generate.argtypes = [i32, i32, i32, i32, d, d, C.c_uint64, d, d, d, d, d,
+                                           vp, vp, vp, vp]
     return L
 
 
@@ -89,7 +122,10 @@ EXPORTED_SYMBOLS = (
     "bicos_hip_release_workspace", "bicos_hip_set_search_impl", "bicos_hip_transform", "bicos_hip_search", "bicos_hip_agree",
     "bicos_hip_agree_subpixel", "bicos_hip_timing_enable", "bicos_hip_timing_read",
     "bicos_hip_malloc", "bicos_hip_free", "bicos_hip_memcpy_h2d", "bicos_hip_memcpy_d2h",
-    "bicos_hip_device_synchronize",
+    "bicos_hip_device_synchronize", "bicos_hip_window_param_count",
+    "bicos_hip_window_options_default", "bicos_hip_window_init", "bicos_hip_window_label",
+    "bicos_hip_window_loss_grad", "bicos_hip_window_adam", "bicos_hip_window_predict",
+    "bicos_hip_match_windowed", "bicos_hip_scene_generate",
 )
 
 

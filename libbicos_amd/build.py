@@ -31,8 +31,9 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument", f"-I{INCLUDE}",
 ]
 
-HIP_SOURCES = ["bicos_hip.hip", "match.cpp"]
-HIP_DEPS = ["device_common.hpp", "transform.hpp", "search.hpp", "search_mfma.hpp", "agree.hpp"]
+HIP_SOURCES = ["bicos_hip.hip", "window.hip", "scene_gpu.hip", "match.cpp"]
+HIP_DEPS = ["device_common.hpp", "host_common.hpp", "transform.hpp", "search.hpp", "search_mfma.hpp",
+            "agree.hpp", "agree_half.hpp", "cnn.hpp", "window.hpp", "scene_field.hpp"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -47,12 +48,24 @@ def _run(cmd: list[str]) -> None:
 
 def build_hip() -> str:
     out = os.path.join(HERE, "libbicos_hip.so")
-    objs = []
-    for src in HIP_SOURCES:
+    objs, procs = [], []
+    for src in HIP_SOURCES:  # translation units compile in parallel
         obj = os.path.join(CSRC, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmd = [HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
+    failed = None
+    for cmd, pr in procs:
+        text, _ = pr.communicate()
+        text = "\n".join(l for l in text.splitlines() if "packed-fp32-ops" not in l)
+        if text.strip():
+            print(text, flush=True)
+        if pr.returncode != 0 and failed is None:
+            failed = cmd
+    if failed:
+        raise RuntimeError(f"build step failed: {' '.join(failed)}")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)
     for o in objs:

@@ -69,13 +69,10 @@ __device__ __forceinline__ void gather(float (&p)[NMAX], const Stack& f,
 // agree (agree.hpp:52-92): in place on the int16 disparity; every pixel of
 // the corrmap is written (NaN where no correlation is evaluated).
 template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
-__global__ __launch_bounds__(256) void k_agree(int16_t* __restrict__ disp, int64_t dstep,
-                                               const Stack f0, const Stack f1, int n_rt,
-                                               int W, float thr, int has_minvar, float minvar,
-                                               TPrec* __restrict__ corrmap, int64_t cstep) {
-    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
-    if (col >= W)
-        return;
+__device__ __forceinline__ void agree_px(int row, int col, int16_t* __restrict__ disp, int64_t dstep,
+                                         const Stack& f0, const Stack& f1, int n_rt, int W,
+                                         float thr, int has_minvar, float minvar,
+                                         TPrec* __restrict__ corrmap, int64_t cstep) {
     const Depth<NMAX, N_EXACT> n{n_rt};
 
     int16_t* dp = disp + (size_t)row * dstep + col;
@@ -96,6 +93,35 @@ __global__ __launch_bounds__(256) void k_agree(int16_t* __restrict__ disp, int64
     }
     if (corrmap)
         corrmap[(size_t)row * cstep + col] = nxc;
+}
+
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
+__global__ __launch_bounds__(256) void k_agree(int16_t* __restrict__ disp, int64_t dstep,
+                                               const Stack f0, const Stack f1, int n_rt,
+                                               int W, float thr, int has_minvar, float minvar,
+                                               TPrec* __restrict__ corrmap, int64_t cstep) {
+    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
+    if (col >= W)
+        return;
+    agree_px<TIn, TPrec, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, has_minvar,
+                                        minvar, corrmap, cstep);
+}
+
+// k_agree over the listed pixels only: entries (row << 16 | col) at
+// list[0 .. *count), grid-stride
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
+__global__ __launch_bounds__(256) void k_agree_list(int16_t* __restrict__ disp, int64_t dstep,
+                                                    const Stack f0, const Stack f1, int n_rt,
+                                                    int W, float thr, int has_minvar, float minvar,
+                                                    TPrec* __restrict__ corrmap, int64_t cstep,
+                                                    const uint32_t* __restrict__ list,
+                                                    const unsigned int* __restrict__ count) {
+    const unsigned int cnt = *count;
+    for (unsigned int i = blockIdx.x * 256 + threadIdx.x; i < cnt; i += gridDim.x * 256) {
+        const uint32_t e = list[i];
+        agree_px<TIn, TPrec, NMAX, N_EXACT>((int)(e >> 16), (int)(e & 0xFFFFu), disp, dstep, f0, f1,
+                                            n_rt, W, thr, has_minvar, minvar, corrmap, cstep);
+    }
 }
 
 // The grid scan of agree_subpixel (agree.hpp:160-172): NCC of the left pixel
@@ -144,13 +170,10 @@ __device__ __forceinline__ void subpixel_scan(const float (&a)[NMAX], const floa
 // variance do not depend on x and are computed once (same values as the
 // reference's per-x recomputation).
 template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
-__global__ __launch_bounds__(256) void k_agree_subpixel(
-    const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1,
-    int n_rt, int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
-    int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep) {
-    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
-    if (col >= W)
-        return;
+__device__ __forceinline__ void agree_subpixel_px(
+    int row, int col, const int16_t* __restrict__ disp, int64_t dstep, const Stack& f0,
+    const Stack& f1, int n_rt, int W, float thr, float step, int has_minvar, float minvar,
+    float* __restrict__ out, int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep) {
     const Depth<NMAX, N_EXACT> n{n_rt};
     const bool mv = has_minvar != 0;
 
@@ -244,6 +267,34 @@ __global__ __launch_bounds__(256) void k_agree_subpixel(
     out[(size_t)row * ostep + col] = res;
     if (corrmap)
         corrmap[(size_t)row * cstep + col] = corr;
+}
+
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
+__global__ __launch_bounds__(256) void k_agree_subpixel(
+    const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1,
+    int n_rt, int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
+    int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep) {
+    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
+    if (col >= W)
+        return;
+    agree_subpixel_px<TIn, TPrec, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, step,
+                                                 has_minvar, minvar, out, ostep, corrmap, cstep);
+}
+
+// k_agree_subpixel over the listed pixels only (entries row << 16 | col)
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
+__global__ __launch_bounds__(256) void k_agree_subpixel_list(
+    const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1,
+    int n_rt, int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
+    int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep, const uint32_t* __restrict__ list,
+    const unsigned int* __restrict__ count) {
+    const unsigned int cnt = *count;
+    for (unsigned int i = blockIdx.x * 256 + threadIdx.x; i < cnt; i += gridDim.x * 256) {
+        const uint32_t e = list[i];
+        agree_subpixel_px<TIn, TPrec, NMAX, N_EXACT>((int)(e >> 16), (int)(e & 0xFFFFu), disp, dstep,
+                                                     f0, f1, n_rt, W, thr, step, has_minvar, minvar,
+                                                     out, ostep, corrmap, cstep);
+    }
 }
 
 }  // namespace bicos_hip

@@ -24,11 +24,12 @@
 #include "agree_half.hpp"
 #include "search.hpp"
 #include "search_mfma.hpp"
+#include "host_common.hpp"
 #include "transform.hpp"
 
 using namespace bicos_hip;
 
-namespace {
+namespace bicos_hip {
 
 thread_local std::string g_error;
 
@@ -37,19 +38,7 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(BICOS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
 /* ----------------------------------------------------------------- timing */
-
-struct TimedLaunch {
-    const char* name;
-    hipEvent_t start, stop;
-};
 
 std::mutex g_timing_mu;
 bool g_timing = false;
@@ -67,34 +56,6 @@ hipEvent_t pooled_event() {
     return e;
 }
 
-template<typename Kernel, typename... Args>
-int launch(const char* name, Kernel kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s,
-           Args... args) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (g_timing) {
-        std::lock_guard<std::mutex> lk(g_timing_mu);
-        e0 = pooled_event();
-        e1 = pooled_event();
-        if (e0) (void)hipEventRecord(e0, s);
-    }
-    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess)
-        return fail(BICOS_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(err));
-    if (e0 && e1) {
-        (void)hipEventRecord(e1, s);
-        std::lock_guard<std::mutex> lk(g_timing_mu);
-        g_timed.push_back({name, e0, e1});
-    }
-    return BICOS_OK;
-}
-
-#define LAUNCH(...)                                \
-    do {                                           \
-        int rc_ = launch(__VA_ARGS__);             \
-        if (rc_ != BICOS_OK) return rc_;           \
-    } while (0)
-
 /* -------------------------------------------------------------- workspace */
 
 struct Workspace {
@@ -107,10 +68,8 @@ struct Workspace {
 std::mutex g_ws_mu;
 std::map<int, Workspace> g_ws;
 
-// Returns a device buffer of at least `bytes` that is free for use on `s`
-// (the stream waits for the previous call that used it). Caller holds g_ws_mu.
-int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out) {
-    Workspace& w = g_ws[dev];
+int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out, int slot) {
+    Workspace& w = g_ws[dev * WS_SLOTS + slot];
     if (!w.last) HIP_TRY(hipEventCreateWithFlags(&w.last, hipEventDisableTiming));
     if (w.cap < bytes) {
         if (w.used) HIP_TRY(hipEventSynchronize(w.last));
@@ -127,14 +86,12 @@ int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out) {
     return BICOS_OK;
 }
 
-int ws_release(int dev, hipStream_t s) {
-    Workspace& w = g_ws[dev];
+int ws_release(int dev, hipStream_t s, int slot) {
+    Workspace& w = g_ws[dev * WS_SLOTS + slot];
     HIP_TRY(hipEventRecord(w.last, s));
     w.used = true;
     return BICOS_OK;
 }
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 /* ------------------------------------------------------------- validation */
 
@@ -162,10 +119,6 @@ int words_for_bits(int bits) {
     if (bits <= 256) return 8;
     return 0;
 }
-
-struct StackInfo {
-    int rows = 0, cols = 0, depth = 0;
-};
 
 int check_stack(const bicos_image* s, uint32_t n, StackInfo* info, FrameTable* ft) {
     if (!s) return fail(BICOS_ERR_EXCEPTION, "null image stack");
@@ -451,6 +404,55 @@ int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stac
         : run_subpixel_t<uint16_t, float>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s);
 }
 
+template<typename TIn, typename TPrec>
+int run_agree_list_t(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, uint32_t n,
+                     int W, float thr, int has_mv, float mv, TPrec* corr, int64_t cstep,
+                     const uint32_t* list, const unsigned int* count, int blocks, hipStream_t s) {
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        LAUNCH("agree_list", (k_agree_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(blocks), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, has_mv, mv, corr, cstep, list, count);
+        return BICOS_OK;
+    });
+}
+
+template<typename TIn, typename TPrec>
+int run_subpixel_list_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                        uint32_t n, int W, float thr, float step, int has_mv, float mv, float* out,
+                        int64_t ostep, TPrec* corr, int64_t cstep, const uint32_t* list,
+                        const unsigned int* count, int blocks, hipStream_t s) {
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        LAUNCH("agree_subpixel_list", (k_agree_subpixel_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(blocks), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep, list, count);
+        return BICOS_OK;
+    });
+}
+
+int run_agree_list(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
+                   uint32_t n, int W, float thr, int has_mv, float mv_n, int precision, void* corr,
+                   int64_t cstep, const uint32_t* list, const unsigned int* count, int blocks,
+                   hipStream_t s) {
+    if (precision == BICOS_PRECISION_DOUBLE)
+        return depth == BICOS_8U
+            ? run_agree_list_t<uint8_t, double>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (double*)corr, cstep, list, count, blocks, s)
+            : run_agree_list_t<uint16_t, double>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (double*)corr, cstep, list, count, blocks, s);
+    return depth == BICOS_8U
+        ? run_agree_list_t<uint8_t, float>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (float*)corr, cstep, list, count, blocks, s)
+        : run_agree_list_t<uint16_t, float>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (float*)corr, cstep, list, count, blocks, s);
+}
+
+int run_subpixel_list(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                      int depth, uint32_t n, int W, float thr, float step, int has_mv, float mv_n,
+                      int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
+                      const uint32_t* list, const unsigned int* count, int blocks, hipStream_t s) {
+    if (precision == BICOS_PRECISION_DOUBLE)
+        return depth == BICOS_8U
+            ? run_subpixel_list_t<uint8_t, double>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, list, count, blocks, s)
+            : run_subpixel_list_t<uint16_t, double>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, list, count, blocks, s);
+    return depth == BICOS_8U
+        ? run_subpixel_list_t<uint8_t, float>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, list, count, blocks, s)
+        : run_subpixel_list_t<uint16_t, float>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, list, count, blocks, s);
+}
+
 // Number of points of the subpixel grid `for (float x = -1; x <= 1; x += step)`
 // (agree.hpp:162), counted with the same float arithmetic the kernel uses;
 // -1 when the step is not positive or so small that the loop would not end
@@ -531,7 +533,7 @@ int with_stacks(const FrameTable* f, const StackInfo* info, int count, uint32_t 
     return rc ? rc : rc2;
 }
 
-}  // namespace
+}  // namespace bicos_hip
 
 /* ================================================================== C ABI */
 
@@ -664,13 +666,15 @@ int bicos_hip_release_workspace(void) {
     int dev, rc;
     if ((rc = current_device(&dev))) return rc;
     std::lock_guard<std::mutex> lk(g_ws_mu);
-    auto it = g_ws.find(dev);
-    if (it == g_ws.end()) return BICOS_OK;
-    Workspace& w = it->second;
-    if (w.used) HIP_TRY(hipEventSynchronize(w.last));
-    if (w.buf) HIP_TRY(hipFree(w.buf));
-    if (w.last) HIP_TRY(hipEventDestroy(w.last));
-    g_ws.erase(it);
+    for (int slot = 0; slot < WS_SLOTS; ++slot) {
+        auto it = g_ws.find(dev * WS_SLOTS + slot);
+        if (it == g_ws.end()) continue;
+        Workspace& w = it->second;
+        if (w.used) HIP_TRY(hipEventSynchronize(w.last));
+        if (w.buf) HIP_TRY(hipFree(w.buf));
+        if (w.last) HIP_TRY(hipEventDestroy(w.last));
+        g_ws.erase(it);
+    }
     return BICOS_OK;
 }
 

@@ -5,7 +5,7 @@
 //   x0[t]  = (4x4 box mean of frame t) - (mean over the 4 frames), times in_scale
 //   a1 = relu(conv3x3(x0, 4 -> 32)), a2 = relu(conv3x3(a1)), a3 = relu(conv3x3(a2)),
 //   f  = conv3x3(a3)                       (32 features per cell)
-//   C[k](Y, X) = <fL(Y, X), fR(Y, X - (D0 + k))> / sqrt(32),  k < K
+//   C[k](Y, X) = <fL(Y, X), fR(Y, X - (D0 + k))> / sqrt(32),  k < K (runtime)
 // The window of a pixel is centred on 4 * (D0 + k*) with k* the refined
 // correlation peak of its cell. Training minimises the cross-entropy between
 // softmax_k C and the plain BICOS disparity of the cell (split linearly over
@@ -23,7 +23,7 @@ namespace bicos_hip {
 constexpr int CNN_C = 32;        // channels
 constexpr int CNN_IN = 4;        // input frames per view
 constexpr int CNN_DS = 4;        // downsampling factor of the grid
-constexpr int CNN_K = 80;        // correlation bins
+constexpr int CNN_K = 80;        // default number of correlation bins (disparities -32 .. 284 px)
 constexpr int CNN_D0 = -8;       // disparity (grid cells) of bin 0
 
 // flat parameter layout: w1 b1 w2 b2 w3 b3 w4 b4, w[co][ci][3][3]
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_cnn_conv(const float* __restrict__ x, c
 
 // correlation volume C[k][Y][X] (masked bins: -1e30)
 __global__ __launch_bounds__(256) void k_cnn_corr(const float* __restrict__ fl, const float* __restrict__ fr,
-                                                  int Hq, int Wq, float* __restrict__ C) {
+                                                  int Hq, int Wq, int K, float* __restrict__ C) {
     const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
     if (X >= Wq)
         return;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_cnn_corr(const float* __restrict__ fl, 
     float l[CNN_C];
 #pragma unroll
     for (int c = 0; c < CNN_C; ++c) l[c] = fl[c * plane + at];
-    for (int k = 0; k < CNN_K; ++k) {
+    for (int k = 0; k < K; ++k) {
         const int xr = X - (CNN_D0 + k);
         float s = -1e30f;
         if (xr >= 0 && xr < Wq) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_cnn_corr(const float* __restrict__ fl, 
 // NaN = unlabelled); writes dC (scaled by 1/count later) and the cell loss
 __global__ __launch_bounds__(256) void k_cnn_softmax_ce(const float* __restrict__ C,
                                                         const float* __restrict__ label, int Hq,
-                                                        int Wq, float* __restrict__ dC,
+                                                        int Wq, int K, float* __restrict__ dC,
                                                         float* __restrict__ cell_loss) {
     const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
     if (X >= Wq)
@@ -147,9 +147,9 @@ __global__ __launch_bounds__(256) void k_cnn_softmax_ce(const float* __restrict_
     const size_t plane = (size_t)Hq * Wq, at = (size_t)Y * Wq + X;
     const float lab = label[at];
     float mx = -1e30f;
-    for (int k = 0; k < CNN_K; ++k) mx = fmaxf(mx, C[k * plane + at]);
+    for (int k = 0; k < K; ++k) mx = fmaxf(mx, C[k * plane + at]);
     float z = 0.0f;
-    for (int k = 0; k < CNN_K; ++k) {
+    for (int k = 0; k < K; ++k) {
         const float c = C[k * plane + at];
         z += c > -1e29f ? __expf(c - mx) : 0.0f;
     }
@@ -157,13 +157,13 @@ __global__ __launch_bounds__(256) void k_cnn_softmax_ce(const float* __restrict_
     float loss = 0.0f;
     const bool has = !__builtin_isnan(lab);
     float pos = has ? lab / CNN_DS - CNN_D0 : 0.0f;   // fractional bin
-    bool in = has && pos >= 0.0f && pos <= (float)(CNN_K - 1);
+    bool in = has && pos >= 0.0f && pos <= (float)(K - 1);
     const int k0 = in ? (int)floorf(pos) : -1;
     const float w1 = in ? pos - k0 : 0.0f;
     // a cell whose target bins fall outside the right image is unlabelled
     const int xr0 = X - (CNN_D0 + k0);
     in = in && xr0 >= 0 && xr0 < Wq && (w1 == 0.0f || xr0 - 1 >= 0);
-    for (int k = 0; k < CNN_K; ++k) {
+    for (int k = 0; k < K; ++k) {
         const float c = C[k * plane + at];
         const float p = c > -1e29f ? __expf(c - lz) : 0.0f;
         const float tgt = !in ? 0.0f : (k == k0 ? 1.0f - w1 : (k == k0 + 1 ? w1 : 0.0f));
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_cnn_softmax_ce(const float* __restrict_
 // gradients of the correlation w.r.t. both feature maps
 __global__ __launch_bounds__(256) void k_cnn_corr_bwd(const float* __restrict__ fl, const float* __restrict__ fr,
                                                       const float* __restrict__ dC, float scale, int Hq,
-                                                      int Wq, float* __restrict__ dfl,
+                                                      int Wq, int K, float* __restrict__ dfl,
                                                       float* __restrict__ dfr) {
     const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
     if (X >= Wq)
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_cnn_corr_bwd(const float* __restrict__ 
     float gl[CNN_C], gr[CNN_C];
 #pragma unroll
     for (int c = 0; c < CNN_C; ++c) gl[c] = gr[c] = 0.0f;
-    for (int k = 0; k < CNN_K; ++k) {
+    for (int k = 0; k < K; ++k) {
         const int d = CNN_D0 + k;
         const int xr = X - d;            // this cell as the left one
         if (xr >= 0 && xr < Wq) {
@@ -293,12 +293,14 @@ __global__ void k_cnn_reduce_weight(const float* __restrict__ partial, int bands
     grad[dst] = acc ? grad[dst] + s : s;
 }
 
+// Adam on the flat parameters; the gradient is grad / max(count, 1)
 __global__ void k_cnn_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                           const float* __restrict__ g, float lr, float b1, float b2, float eps,
-                           float c1, float c2) {
+                           const float* __restrict__ g, const float* __restrict__ count, float lr,
+                           float b1, float b2, float eps, float c1, float c2) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= CNN_PARAMS) return;
-    const float gi = g[i];
+    const float scale = 1.0f / fmaxf(count ? count[0] : 1.0f, 1.0f);
+    const float gi = g[i] * scale;
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -306,49 +308,42 @@ __global__ void k_cnn_adam(float* __restrict__ p, float* __restrict__ m, float* 
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// window of every full-resolution pixel from its cell's correlation peak:
-// centre = 4 (D0 + k*) refined by a parabola through the peak bins,
-// half-width from the peak's softmax probability (hw_hi below p_conf)
-__global__ __launch_bounds__(256) void k_cnn_window(const float* __restrict__ C, int Hq, int Wq,
-                                                    int H, int W, float p_conf, int hw_lo,
-                                                    int hw_hi, float* __restrict__ center,
-                                                    int16_t* __restrict__ halfwidth) {
+// window of every cell from its correlation peak: centre 4 (D0 + k*) px
+// with k* the argmax bin refined by a parabola through its neighbours,
+// half-width hw_lo where the peak's softmax probability is at least p_conf,
+// else hw_hi
+__global__ __launch_bounds__(256) void k_cnn_window(const float* __restrict__ C, int Hq, int Wq, int K,
+                                                    float p_conf, int hw_lo, int hw_hi,
+                                                    float* __restrict__ center,
+                                                    int16_t* __restrict__ halfwidth,
+                                                    float* __restrict__ peak_prob) {
     const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
     if (X >= Wq)
         return;
     const size_t plane = (size_t)Hq * Wq, at = (size_t)Y * Wq + X;
     int kb = 0;
     float mx = -1e30f;
-    for (int k = 0; k < CNN_K; ++k) {
+    for (int k = 0; k < K; ++k) {
         const float c = C[k * plane + at];
         if (c > mx) { mx = c; kb = k; }
     }
     float z = 0.0f;
-    for (int k = 0; k < CNN_K; ++k) {
+    for (int k = 0; k < K; ++k) {
         const float c = C[k * plane + at];
         z += c > -1e29f ? __expf(c - mx) : 0.0f;
     }
     const float pk = 1.0f / z;
     float kf = (float)kb;
-    if (kb > 0 && kb < CNN_K - 1) {
+    if (kb > 0 && kb < K - 1) {
         const float cm = C[(kb - 1) * plane + at], cp = C[(kb + 1) * plane + at];
         if (cm > -1e29f && cp > -1e29f) {
             const float den = cm - 2.0f * mx + cp;
             if (den < 0.0f) kf += fminf(fmaxf(0.5f * (cm - cp) / den, -0.5f), 0.5f);
         }
     }
-    const float c = CNN_DS * (CNN_D0 + kf);
-    const int16_t hw = (int16_t)(pk >= p_conf ? hw_lo : hw_hi);
-    for (int dy = 0; dy < CNN_DS; ++dy) {
-        const int y = Y * CNN_DS + dy;
-        if (y >= H) continue;
-        for (int dx = 0; dx < CNN_DS; ++dx) {
-            const int x = X * CNN_DS + dx;
-            if (x >= W) continue;
-            center[(size_t)y * W + x] = c;
-            halfwidth[(size_t)y * W + x] = hw;
-        }
-    }
+    center[at] = CNN_DS * (CNN_D0 + kf);
+    halfwidth[at] = (int16_t)(pk >= p_conf ? hw_lo : hw_hi);
+    if (peak_prob) peak_prob[at] = pk;
 }
 
 }  // namespace bicos_hip

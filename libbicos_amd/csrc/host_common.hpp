@@ -1,0 +1,123 @@
+// Host-side helpers shared by the translation units of libbicos_hip.so:
+// error reporting, timed launches, the per-device workspace, input
+// validation and the stage dispatchers of the plain matching path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bicos_hip.h"
+#include "device_common.hpp"
+
+namespace bicos_hip {
+
+extern thread_local std::string g_error;
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(BICOS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+/* ----------------------------------------------------------------- timing */
+
+struct TimedLaunch {
+    const char* name;
+    hipEvent_t start, stop;
+};
+
+extern std::mutex g_timing_mu;
+extern bool g_timing;
+extern std::vector<TimedLaunch> g_timed;
+hipEvent_t pooled_event();
+
+template<typename Kernel, typename... Args>
+int launch(const char* name, Kernel kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+           Args... args) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_timing) {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        e0 = pooled_event();
+        e1 = pooled_event();
+        if (e0) (void)hipEventRecord(e0, s);
+    }
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+        return fail(BICOS_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(err));
+    if (e0 && e1) {
+        (void)hipEventRecord(e1, s);
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        g_timed.push_back({name, e0, e1});
+    }
+    return BICOS_OK;
+}
+
+#define LAUNCH(...)                                \
+    do {                                           \
+        int rc_ = launch(__VA_ARGS__);             \
+        if (rc_ != BICOS_OK) return rc_;           \
+    } while (0)
+
+/* ------------------------------------------------ workspace, validation */
+
+// Device buffers kept between calls, one per (device, slot); guarded by
+// g_ws_mu. ws_acquire returns a buffer of at least `bytes` that is free for
+// use on `s` (the stream waits for the last call that used it); ws_release
+// marks the end of that use on `s`.
+enum WorkspaceSlot : int { WS_MATCH = 0, WS_SCENE = 1, WS_WINDOW = 2, WS_SLOTS = 3 };
+extern std::mutex g_ws_mu;
+int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out, int slot = WS_MATCH);
+int ws_release(int dev, hipStream_t s, int slot = WS_MATCH);
+int current_device(int* dev);
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct StackInfo {
+    int rows = 0, cols = 0, depth = 0;
+};
+
+int elem_size(int type);
+int required_bits(uint32_t n, int mode);
+int words_for_bits(int bits);
+int check_stack(const bicos_image* s, uint32_t n, StackInfo* info, FrameTable* ft);
+bool uniformly_spaced(const FrameTable& f, uint32_t n);
+size_t packed_bytes(const FrameTable& f, uint32_t n, const StackInfo& i);
+int make_stack(const FrameTable& f, uint32_t n, const StackInfo& i, void* scratch, Stack* out,
+               hipStream_t s);
+int check_step(float step);
+int subpixel_points(float step);
+int check_out(const bicos_image* img, int rows, int cols, int type, const char* what);
+
+/* ------------------------------------------------------ stage dispatch */
+
+int transform_any(const Stack& f0, const Stack* f1, int depth, uint32_t n, int mode,
+                  int W, int H, int words, uint32_t* out0, uint32_t* out1, hipStream_t s);
+int search_flags(const uint32_t* d0, const uint32_t* d1, int H, int W, int words, int flags,
+                 int max_lr, int16_t* out, int64_t ostep, int16_t* rev_scratch, hipStream_t s);
+int run_agree(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
+              uint32_t n, int W, int H, float thr, int has_mv, float mv_n, int precision,
+              void* corr, int64_t cstep, hipStream_t s);
+int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                 int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                 float mv_n, int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
+                 hipStream_t s);
+// NCC stage of the listed pixels only (entries row << 16 | col at
+// list[0 .. *count), `blocks` x 256 lanes, grid-stride): exact FP32
+// arithmetic, FP64 for PRECISION_DOUBLE.
+int run_agree_list(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
+                   uint32_t n, int W, float thr, int has_mv, float mv_n, int precision, void* corr,
+                   int64_t cstep, const uint32_t* list, const unsigned int* count, int blocks,
+                   hipStream_t s);
+int run_subpixel_list(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                      int depth, uint32_t n, int W, float thr, float step, int has_mv, float mv_n,
+                      int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
+                      const uint32_t* list, const unsigned int* count, int blocks, hipStream_t s);
+
+}  // namespace bicos_hip

@@ -33,29 +33,11 @@
 #include <thread>
 #include <vector>
 
+#include "scene_field.hpp"
+
 namespace {
 
-inline uint64_t splitmix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-inline uint64_t hash5(uint64_t seed, uint64_t stream, uint64_t t, uint64_t y, uint64_t x) {
-    uint64_t h = splitmix64(seed);
-    h = splitmix64(h ^ (stream * 0xD1B54A32D192ED03ull));
-    h = splitmix64(h ^ (t * 0x8CB92BA72F3D8DD7ull));
-    h = splitmix64(h ^ (y * 0xABC98388FB8FAC03ull));
-    h = splitmix64(h ^ x);
-    return h;
-}
-
-/* uniform in [0, 1) with 24 random bits */
-inline float u01(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
-inline double u01d(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
-
-enum Stream : uint64_t { S_PARAM = 1, S_SPECKLE = 2, S_NOISE_L = 3, S_NOISE_R = 4, S_DARK = 5 };
+using namespace bicos_scene;
 
 template<typename F>
 void parallel_for(int count, int threads, F&& f) {
@@ -67,50 +49,6 @@ void parallel_for(int count, int threads, F&& f) {
             for (int k; (k = next.fetch_add(1)) < count;) f(k);
         });
     for (auto& th : pool) th.join();
-}
-
-struct Field {
-    double dir[4], period[4], amp[4], phase[4];
-    double px[2], py[2], nx[2], ny[2], h[2];
-};
-
-Field make_field(uint64_t seed, int W, int H) {
-    Field f;
-    int k = 0;
-    auto r = [&]() { return u01d(hash5(seed, S_PARAM, 0, 0, (uint64_t)k++)); };
-    for (int i = 0; i < 4; ++i) {
-        f.dir[i] = r() * 2.0 * M_PI;
-        f.period[i] = W * (0.25 + 1.75 * r());
-        f.amp[i] = 0.5 + 0.5 * r();
-        f.phase[i] = r() * 2.0 * M_PI;
-    }
-    for (int j = 0; j < 2; ++j) {
-        f.px[j] = r() * W;
-        f.py[j] = r() * H;
-        double a = r() * 2.0 * M_PI;
-        f.nx[j] = std::cos(a);
-        f.ny[j] = std::sin(a);
-        f.h[j] = (0.4 + 0.6 * r()) * (r() < 0.5 ? -1.0 : 1.0);
-    }
-    return f;
-}
-
-double field_value(const Field& f, double x, double y) {
-    double v = 0.0;
-    for (int i = 0; i < 4; ++i) {
-        double u = x * std::cos(f.dir[i]) + y * std::sin(f.dir[i]);
-        v += f.amp[i] * std::sin(2.0 * M_PI * u / f.period[i] + f.phase[i]);
-    }
-    for (int j = 0; j < 2; ++j)
-        if ((x - f.px[j]) * f.nx[j] + (y - f.py[j]) * f.ny[j] > 0.0) v += f.h[j];
-    return v;
-}
-
-double gauss_noise(uint64_t h) {
-    /* Box-Muller from two 32-bit halves of one hash */
-    double u1 = ((double)(h >> 32) + 0.5) * (1.0 / 4294967296.0);
-    double u2 = ((double)(h & 0xFFFFFFFFull) + 0.5) * (1.0 / 4294967296.0);
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
 }
 
 template<typename T>
@@ -162,10 +100,9 @@ void generate(int W, int H, int n, double dmin, double dmax, uint64_t seed, doub
     });
 
     const int CW = W + (int)std::ceil(dmax) + 8; /* canvas width */
-    const int R = 5;
-    double g[2 * R + 1], gs = 0.0;
-    for (int i = -R; i <= R; ++i) gs += g[i + R] = std::exp(-(double)(i * i) / (2.0 * 1.5 * 1.5));
-    for (double& v : g) v /= gs;
+    const int R = BLUR_R;
+    double g[2 * R + 1];
+    blur_taps(g);
     const double vmax = (double)std::numeric_limits<T>::max();
 
     parallel_for(n, threads, [&](int t) {

@@ -1,0 +1,554 @@
+// Learned search window: CNN training and inference entry points and the
+// windowed matching path (include/bicos_hip.h, "learned search window").
+//
+// bicos_hip_match_windowed runs BICOS::match (src/impl/cuda.cu.in:54-292
+// pipeline) with the full-row search replaced by
+//   CNN windows -> windowed search -> NCC stage -> fallback list ->
+//   full-row search of the listed pixels -> NCC stage of the listed pixels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "host_common.hpp"
+#include "window.hpp"
+
+using namespace bicos_hip;
+
+namespace bicos_hip {
+namespace {
+
+constexpr int CNN_BANDS_ROWS = 8;  // cell rows per weight-gradient band
+
+struct Grid {
+    int H, W, Hq, Wq, K;
+    size_t cells;
+};
+
+int check_options(const bicos_window_options* o, int depth, float* in_scale, int* K) {
+    if (!o) return fail(BICOS_ERR_INVALID_ARGUMENT, "null window options");
+    if (o->bins < 9 || o->bins > 1024)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "window bins must be in [9, 1024]");
+    if (o->hw_lo < 0 || o->hw_hi < 0 || o->hw_lo > 32767 || o->hw_hi > 32767)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "window half-widths must be in [0, 32767]");
+    if (!(o->in_scale >= 0.0f) || !std::isfinite(o->in_scale))
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "bad input scale");
+    *in_scale = o->in_scale > 0.0f ? o->in_scale : 8.0f / (depth == BICOS_8U ? 255.0f : 65535.0f);
+    *K = o->bins;
+    return BICOS_OK;
+}
+
+int check_pair(const bicos_image* stack0, const bicos_image* stack1, uint32_t n, StackInfo* i0,
+               StackInfo* i1, FrameTable* f0, FrameTable* f1) {
+    int rc;
+    if ((rc = check_stack(stack0, n, i0, f0))) return rc;
+    if ((rc = check_stack(stack1, n, i1, f1))) return rc;
+    if (i0->rows != i1->rows || i0->cols != i1->cols || i0->depth != i1->depth)
+        return fail(BICOS_ERR_EXCEPTION, "the two stacks differ in size or type");
+    if (n < (uint32_t)CNN_IN)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "the learned window needs at least 4 frames");
+    if (i0->rows < CNN_DS || i0->cols < CNN_DS)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "the learned window needs at least 4 rows and columns");
+    if (i0->rows > 65535) return fail(BICOS_ERR_INVALID_ARGUMENT, "at most 65535 rows");
+    return BICOS_OK;
+}
+
+/* --------------------------------------------------------------- network */
+
+struct ViewActs {
+    float *x0, *a1, *a2, *a3, *f;  // 4, 32, 32, 32, 32 planes of cells
+};
+
+size_t acts_floats(size_t cells) { return (size_t)(CNN_IN + 4 * CNN_C) * cells; }
+
+ViewActs carve_acts(float* p, size_t cells) {
+    ViewActs a;
+    a.x0 = p;
+    a.a1 = a.x0 + CNN_IN * cells;
+    a.a2 = a.a1 + CNN_C * cells;
+    a.a3 = a.a2 + CNN_C * cells;
+    a.f = a.a3 + CNN_C * cells;
+    return a;
+}
+
+int run_bf16_weights(const float* params, float* wbf, hipStream_t s) {
+    LAUNCH("cnn_weights", k_cnn_bf16_weights, dim3((CNN_PARAMS + 255) / 256), dim3(256), 0, s, params, wbf);
+    return BICOS_OK;
+}
+
+int cnn_forward(const Stack& f, int depth, const Grid& g, float in_scale, const float* wbf,
+                const float* params, const ViewActs& a, hipStream_t s) {
+    const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
+    if (depth == BICOS_8U)
+        LAUNCH("cnn_input", k_cnn_input<uint8_t>, grid, block, 0, s, f, g.W, g.Hq, g.Wq, in_scale, a.x0);
+    else
+        LAUNCH("cnn_input", k_cnn_input<uint16_t>, grid, block, 0, s, f, g.W, g.Hq, g.Wq, in_scale, a.x0);
+    LAUNCH("cnn_conv", (k_cnn_conv<CNN_IN, true>), grid, block, 0, s, (const float*)a.x0,
+           wbf + cnn_w_off(0), params + cnn_b_off(0), g.Hq, g.Wq, a.a1);
+    LAUNCH("cnn_conv", (k_cnn_conv<CNN_C, true>), grid, block, 0, s, (const float*)a.a1,
+           wbf + cnn_w_off(1), params + cnn_b_off(1), g.Hq, g.Wq, a.a2);
+    LAUNCH("cnn_conv", (k_cnn_conv<CNN_C, true>), grid, block, 0, s, (const float*)a.a2,
+           wbf + cnn_w_off(2), params + cnn_b_off(2), g.Hq, g.Wq, a.a3);
+    LAUNCH("cnn_conv", (k_cnn_conv<CNN_C, false>), grid, block, 0, s, (const float*)a.a3,
+           wbf + cnn_w_off(3), params + cnn_b_off(3), g.Hq, g.Wq, a.f);
+    return BICOS_OK;
+}
+
+template<int CIN>
+int weight_grad(const float* x, const float* dy, const Grid& g, int layer, float* partial,
+                float* grad, hipStream_t s) {
+    constexpr int NT = CIN * 9 + 1;
+    const int bands = (g.Hq + CNN_BANDS_ROWS - 1) / CNN_BANDS_ROWS;
+    LAUNCH("cnn_weight_grad", k_cnn_conv_bwd_weight<CIN>, dim3(CNN_C, bands), dim3(256), 0, s, x, dy,
+           g.Hq, g.Wq, CNN_BANDS_ROWS, partial);
+    LAUNCH("cnn_weight_grad", k_cnn_reduce_weight<CIN>, dim3((CNN_C * NT + 255) / 256), dim3(256), 0, s,
+           (const float*)partial, bands, layer, 1, grad);
+    return BICOS_OK;
+}
+
+// backward pass of one view from d (gradient w.r.t. its features; used as
+// scratch), accumulating into grad
+int cnn_backward(const ViewActs& a, float* d, float* e, const float* wbf, const Grid& g,
+                 float* partial, float* grad, hipStream_t s) {
+    const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
+    const size_t n32 = (size_t)CNN_C * g.cells;
+    const dim3 gflat((unsigned)((n32 + 255) / 256));
+    int rc;
+    // layer 3 (no ReLU after it)
+    if ((rc = weight_grad<CNN_C>(a.a3, d, g, 3, partial, grad, s))) return rc;
+    LAUNCH("cnn_data_grad", k_cnn_conv_bwd_data<CNN_C>, grid, block, 0, s, (const float*)d, wbf + cnn_w_off(3), g.Hq, g.Wq, e);
+    LAUNCH("cnn_relu_grad", k_cnn_relu_bwd, gflat, block, 0, s, (const float*)a.a3, e, n32);
+    // layer 2
+    if ((rc = weight_grad<CNN_C>(a.a2, e, g, 2, partial, grad, s))) return rc;
+    LAUNCH("cnn_data_grad", k_cnn_conv_bwd_data<CNN_C>, grid, block, 0, s, (const float*)e, wbf + cnn_w_off(2), g.Hq, g.Wq, d);
+    LAUNCH("cnn_relu_grad", k_cnn_relu_bwd, gflat, block, 0, s, (const float*)a.a2, d, n32);
+    // layer 1
+    if ((rc = weight_grad<CNN_C>(a.a1, d, g, 1, partial, grad, s))) return rc;
+    LAUNCH("cnn_data_grad", k_cnn_conv_bwd_data<CNN_C>, grid, block, 0, s, (const float*)d, wbf + cnn_w_off(1), g.Hq, g.Wq, e);
+    LAUNCH("cnn_relu_grad", k_cnn_relu_bwd, gflat, block, 0, s, (const float*)a.a1, e, n32);
+    // layer 0
+    return weight_grad<CNN_IN>(a.x0, e, g, 0, partial, grad, s);
+}
+
+size_t partial_floats(const Grid& g) {
+    return (size_t)((g.Hq + CNN_BANDS_ROWS - 1) / CNN_BANDS_ROWS) * CNN_C * (CNN_C * 9 + 1);
+}
+
+// Windows of all cells into center/halfwidth (peak nullable); the buffers
+// after `p` hold wbf, both views' activations and the correlation volume.
+int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, float in_scale,
+                const float* params, const bicos_window_options* o, float* scratch, float* center,
+                int16_t* halfwidth, float* peak, hipStream_t s) {
+    float* wbf = scratch;
+    float* actp = wbf + align_up(CNN_PARAMS, 64);
+    const ViewActs aL = carve_acts(actp, g.cells);
+    const ViewActs aR = carve_acts(actp + acts_floats(g.cells), g.cells);
+    float* C = actp + 2 * acts_floats(g.cells);
+    int rc;
+    if ((rc = run_bf16_weights(params, wbf, s))) return rc;
+    if ((rc = cnn_forward(v0, depth, g, in_scale, wbf, params, aL, s))) return rc;
+    if ((rc = cnn_forward(v1, depth, g, in_scale, wbf, params, aR, s))) return rc;
+    const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
+    LAUNCH("cnn_corr", k_cnn_corr, grid, block, 0, s, (const float*)aL.f, (const float*)aR.f, g.Hq, g.Wq, g.K, C);
+    LAUNCH("cnn_window", k_cnn_window, grid, block, 0, s, (const float*)C, g.Hq, g.Wq, g.K, o->p_conf,
+           o->hw_lo, o->hw_hi, center, halfwidth, peak);
+    return BICOS_OK;
+}
+
+size_t windows_scratch_floats(const Grid& g) {
+    return align_up(CNN_PARAMS, 64) + 2 * acts_floats(g.cells) + (size_t)g.K * g.cells;
+}
+
+/* ------------------------------------------------------- windowed search */
+
+template<int WORDS>
+int run_window_search_w(const uint32_t* d0, const uint32_t* d1, const Grid& g, bool nodupes,
+                        bool consist, int max_lr, const float* center, const int16_t* hw,
+                        int16_t* out, int64_t ostep, hipStream_t s) {
+    const dim3 grid((g.W + 255) / 256, g.H), block(256);
+#define WS(ND, CS) LAUNCH("window_search", (k_window_search<WORDS, ND, CS>), grid, block, 0, s, d0, d1, g.W, center, hw, g.Hq, g.Wq, max_lr, out, ostep)
+    if (nodupes) {
+        if (consist) WS(true, true); else WS(true, false);
+    } else {
+        if (consist) WS(false, true); else WS(false, false);
+    }
+#undef WS
+    return BICOS_OK;
+}
+
+int run_window_search(const uint32_t* d0, const uint32_t* d1, const Grid& g, int words,
+                      bool nodupes, bool consist, int max_lr, const float* center,
+                      const int16_t* hw, int16_t* out, int64_t ostep, hipStream_t s) {
+    switch (words) {
+        case 1: return run_window_search_w<1>(d0, d1, g, nodupes, consist, max_lr, center, hw, out, ostep, s);
+        case 2: return run_window_search_w<2>(d0, d1, g, nodupes, consist, max_lr, center, hw, out, ostep, s);
+        case 4: return run_window_search_w<4>(d0, d1, g, nodupes, consist, max_lr, center, hw, out, ostep, s);
+        case 8: return run_window_search_w<8>(d0, d1, g, nodupes, consist, max_lr, center, hw, out, ostep, s);
+        default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+    }
+}
+
+template<int WORDS>
+int run_search_list_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes, int mode,
+                      const int16_t* qcol, const int* row_count, int16_t* res, int16_t* disp,
+                      int64_t dstep, hipStream_t s) {
+    const int chunk = (int)std::min<size_t>((size_t)W, 64 * 1024 / (WORDS * 4));
+    const size_t lds = (size_t)chunk * WORDS * 4;
+    const dim3 grid((W + SEARCH_THREADS - 1) / SEARCH_THREADS, H), block(SEARCH_THREADS);
+#define SL(ND, M) LAUNCH("search_list", (k_search_list<WORDS, ND, M>), grid, block, lds, s, dq, dc, W, chunk, qcol, row_count, res, disp, dstep)
+    if (nodupes) {
+        if (mode == SEARCH_REV) SL(true, SEARCH_REV); else SL(true, SEARCH_DISP);
+    } else {
+        if (mode == SEARCH_REV) SL(false, SEARCH_REV); else SL(false, SEARCH_DISP);
+    }
+#undef SL
+    return BICOS_OK;
+}
+
+int run_search_list(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
+                    int mode, const int16_t* qcol, const int* row_count, int16_t* res,
+                    int16_t* disp, int64_t dstep, hipStream_t s) {
+    switch (words) {
+        case 1: return run_search_list_w<1>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+        case 2: return run_search_list_w<2>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+        case 4: return run_search_list_w<4>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+        case 8: return run_search_list_w<8>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+        default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int count) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) p[i] = 0u;
+}
+
+int run_fallback_list(const float* disp_f, const int16_t* disp_i, int64_t dstep, const Stack& v0,
+                      int depth, uint32_t n, const Grid& g, int has_mv, float mv_n, int precision,
+                      int16_t* list, int* row_count, uint32_t* glist, unsigned int* gcount,
+                      hipStream_t s) {
+    LAUNCH("fallback_list", k_zero_u32, dim3(1), dim3(256), 0, s, gcount, 1);
+    const dim3 grid(g.H), block(256);
+#define FL(TI, TP) LAUNCH("fallback_list", (k_fallback_list<TI, TP>), grid, block, 0, s, disp_f, disp_i, dstep, v0, (int)n, g.W, has_mv, mv_n, list, row_count, glist, gcount)
+    if (precision == BICOS_PRECISION_DOUBLE) {
+        if (depth == BICOS_8U) FL(uint8_t, double); else FL(uint16_t, double);
+    } else {
+        if (depth == BICOS_8U) FL(uint8_t, float); else FL(uint16_t, float);
+    }
+#undef FL
+    return BICOS_OK;
+}
+
+}  // namespace
+}  // namespace bicos_hip
+
+/* ================================================================== C ABI */
+
+extern "C" {
+
+int bicos_hip_window_param_count(void) { return CNN_PARAMS; }
+
+void bicos_hip_window_options_default(bicos_window_options* o) {
+    if (!o) return;
+    o->in_scale = 0.0f;
+    o->p_conf = 0.5f;
+    o->hw_lo = 6;
+    o->hw_hi = 12;
+    o->bins = CNN_K;
+    o->fallback = 1;
+}
+
+int bicos_hip_window_init(float* p, uint64_t seed) {
+    if (!p) return fail(BICOS_ERR_INVALID_ARGUMENT, "null parameter buffer");
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < 4; ++l) {
+        const int cin = l == 0 ? CNN_IN : CNN_C;
+        std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / (cin * 9)));
+        for (int i = 0; i < CNN_C * cin * 9; ++i) p[cnn_w_off(l) + i] = (float)nd(rng);
+        for (int i = 0; i < CNN_C; ++i) p[cnn_b_off(l) + i] = 0.0f;
+    }
+    return BICOS_OK;
+}
+
+int bicos_hip_window_label(const bicos_image* disparity, int32_t min_valid, float* label,
+                           void* stream) {
+    if (!disparity || !disparity->data || !label) return fail(BICOS_ERR_INVALID_ARGUMENT, "null buffer");
+    if (disparity->type != BICOS_32F && disparity->type != BICOS_16S)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "disparity must be BICOS_32F or BICOS_16S");
+    if (disparity->rows < CNN_DS || disparity->cols < CNN_DS)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "disparity smaller than one cell");
+    const int es = elem_size(disparity->type);
+    if (disparity->step % es || disparity->step < (int64_t)disparity->cols * es)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "bad row step");
+    const int Hq = disparity->rows / CNN_DS, Wq = disparity->cols / CNN_DS;
+    const bool f = disparity->type == BICOS_32F;
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH("window_label", k_window_label, dim3((Wq + 255) / 256, Hq), dim3(256), 0, s,
+           f ? (const float*)disparity->data : nullptr, f ? nullptr : (const int16_t*)disparity->data,
+           disparity->step / es, Hq, Wq, (int)min_valid, label);
+    return BICOS_OK;
+}
+
+int bicos_hip_window_loss_grad(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                               const float* params, const bicos_window_options* opts,
+                               const float* label, float* grad, float* loss, void* stream) {
+    StackInfo i0, i1;
+    FrameTable f0, f1;
+    int rc;
+    if ((rc = check_pair(stack0, stack1, n, &i0, &i1, &f0, &f1))) return rc;
+    if (!params || !label || !grad || !loss) return fail(BICOS_ERR_INVALID_ARGUMENT, "null buffer");
+    float in_scale;
+    Grid g;
+    if ((rc = check_options(opts, i0.depth, &in_scale, &g.K))) return rc;
+    g.H = i0.rows; g.W = i0.cols; g.Hq = g.H / CNN_DS; g.Wq = g.W / CNN_DS;
+    g.cells = (size_t)g.Hq * g.Wq;
+    hipStream_t s = (hipStream_t)stream;
+
+    const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
+    const size_t floats = align_up(CNN_PARAMS, 64) + 2 * acts_floats(g.cells) +
+                          2 * (size_t)g.K * g.cells + 2 * g.cells + 4 * (size_t)CNN_C * g.cells +
+                          partial_floats(g) + 64;
+    int dev;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    void* ws;
+    if ((rc = ws_acquire(dev, floats * 4 + pk0 + pk1 + 1024, s, &ws, WS_WINDOW))) return rc;
+    auto body = [&]() -> int {
+        int r;
+        uint8_t* pb = (uint8_t*)ws;
+        Stack v0, v1;
+        if ((r = make_stack(f0, n, i0, pb, &v0, s))) return r;
+        if ((r = make_stack(f1, n, i1, pb + pk0, &v1, s))) return r;
+        float* p = (float*)(pb + pk0 + pk1);
+        float* wbf = p; p += align_up(CNN_PARAMS, 64);
+        const ViewActs aL = carve_acts(p, g.cells); p += acts_floats(g.cells);
+        const ViewActs aR = carve_acts(p, g.cells); p += acts_floats(g.cells);
+        float* C = p; p += (size_t)g.K * g.cells;
+        float* dC = p; p += (size_t)g.K * g.cells;
+        float* cell_loss = p; p += 2 * g.cells;
+        float* dL = p; p += (size_t)CNN_C * g.cells;
+        float* dR = p; p += (size_t)CNN_C * g.cells;
+        float* e0 = p; p += (size_t)CNN_C * g.cells;
+        float* e1 = p; p += (size_t)CNN_C * g.cells;
+        float* partial = p;
+        if ((r = run_bf16_weights(params, wbf, s))) return r;
+        if ((r = cnn_forward(v0, i0.depth, g, in_scale, wbf, params, aL, s))) return r;
+        if ((r = cnn_forward(v1, i0.depth, g, in_scale, wbf, params, aR, s))) return r;
+        const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
+        LAUNCH("cnn_corr", k_cnn_corr, grid, block, 0, s, (const float*)aL.f, (const float*)aR.f, g.Hq, g.Wq, g.K, C);
+        LAUNCH("cnn_loss", k_cnn_softmax_ce, grid, block, 0, s, (const float*)C, label, g.Hq, g.Wq, g.K, dC, cell_loss);
+        LAUNCH("cnn_loss", k_loss_sum, dim3(1), block, 0, s, (const float*)cell_loss, g.cells, loss);
+        LAUNCH("cnn_corr_grad", k_cnn_corr_bwd, grid, block, 0, s, (const float*)aL.f, (const float*)aR.f,
+               (const float*)dC, 1.0f, g.Hq, g.Wq, g.K, dL, dR);
+        if ((r = cnn_backward(aL, dL, e0, wbf, g, partial, grad, s))) return r;
+        return cnn_backward(aR, dR, e1, wbf, g, partial, grad, s);
+    };
+    rc = body();
+    const int rc2 = ws_release(dev, s, WS_WINDOW);
+    return rc ? rc : rc2;
+}
+
+int bicos_hip_window_adam(float* params, float* m, float* v, const float* grad, const float* count,
+                          float lr, float beta1, float beta2, float eps, int32_t step,
+                          void* stream) {
+    if (!params || !m || !v || !grad) return fail(BICOS_ERR_INVALID_ARGUMENT, "null buffer");
+    if (step < 1) return fail(BICOS_ERR_INVALID_ARGUMENT, "Adam steps count from 1");
+    const float c1 = 1.0f - std::pow(beta1, (float)step), c2 = 1.0f - std::pow(beta2, (float)step);
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH("cnn_adam", k_cnn_adam, dim3((CNN_PARAMS + 255) / 256), dim3(256), 0, s, params, m, v, grad,
+           count, lr, beta1, beta2, eps, c1, c2);
+    return BICOS_OK;
+}
+
+int bicos_hip_window_predict(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                             const float* params, const bicos_window_options* opts, float* center,
+                             int16_t* halfwidth, float* peak, void* stream) {
+    StackInfo i0, i1;
+    FrameTable f0, f1;
+    int rc;
+    if ((rc = check_pair(stack0, stack1, n, &i0, &i1, &f0, &f1))) return rc;
+    if (!params || !center || !halfwidth) return fail(BICOS_ERR_INVALID_ARGUMENT, "null buffer");
+    float in_scale;
+    Grid g;
+    if ((rc = check_options(opts, i0.depth, &in_scale, &g.K))) return rc;
+    g.H = i0.rows; g.W = i0.cols; g.Hq = g.H / CNN_DS; g.Wq = g.W / CNN_DS;
+    g.cells = (size_t)g.Hq * g.Wq;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
+    int dev;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    void* ws;
+    if ((rc = ws_acquire(dev, windows_scratch_floats(g) * 4 + pk0 + pk1 + 1024, s, &ws, WS_WINDOW))) return rc;
+    auto body = [&]() -> int {
+        int r;
+        uint8_t* pb = (uint8_t*)ws;
+        Stack v0, v1;
+        if ((r = make_stack(f0, n, i0, pb, &v0, s))) return r;
+        if ((r = make_stack(f1, n, i1, pb + pk0, &v1, s))) return r;
+        return cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, (float*)(pb + pk0 + pk1),
+                           center, halfwidth, peak, s);
+    };
+    rc = body();
+    const int rc2 = ws_release(dev, s, WS_WINDOW);
+    return rc ? rc : rc2;
+}
+
+int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                             const bicos_config* cfg, const float* params,
+                             const bicos_window_options* opts, bicos_image* disparity,
+                             bicos_image* corrmap, bicos_window_stats* stats, void* stream) {
+    if (!cfg) return fail(BICOS_ERR_INVALID_ARGUMENT, "null config");
+    if (!params) return fail(BICOS_ERR_INVALID_ARGUMENT, "null window parameters");
+    StackInfo i0, i1;
+    FrameTable f0, f1;
+    int rc;
+    if ((rc = check_pair(stack0, stack1, n, &i0, &i1, &f0, &f1))) return rc;
+    if (cfg->mode != BICOS_MODE_LIMITED && cfg->mode != BICOS_MODE_FULL)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown transform mode");
+    if (cfg->variant != BICOS_VARIANT_NO_DUPLICATES && cfg->variant != BICOS_VARIANT_CONSISTENCY)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented");
+    if (cfg->precision != BICOS_PRECISION_SINGLE && cfg->precision != BICOS_PRECISION_DOUBLE &&
+        cfg->precision != BICOS_PRECISION_HALF_MFMA)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
+    float in_scale;
+    Grid g;
+    if ((rc = check_options(opts, i0.depth, &in_scale, &g.K))) return rc;
+    const int bits = required_bits(n, cfg->mode);
+    const int words = words_for_bits(bits);
+    if (!words)
+        return fail(BICOS_ERR_INVALID_ARGUMENT,
+                    "input stacks too large, would require " + std::to_string(bits) + " bits");
+    const bool thr = cfg->has_nxcorr_threshold != 0;
+    const bool subpix = thr && cfg->has_subpixel_step;
+    if (subpix && (rc = check_step(cfg->subpixel_step))) return rc;
+    g.H = i0.rows; g.W = i0.cols; g.Hq = g.H / CNN_DS; g.Wq = g.W / CNN_DS;
+    g.cells = (size_t)g.Hq * g.Wq;
+    const int H = g.H, W = g.W;
+    if ((rc = check_out(disparity, H, W, bicos_hip_disparity_type(cfg), "disparity"))) return rc;
+    if (corrmap && thr && (rc = check_out(corrmap, H, W, bicos_hip_corrmap_type(cfg), "corrmap")))
+        return rc;
+    hipStream_t s = (hipStream_t)stream;
+
+    const bool consistency = cfg->variant == BICOS_VARIANT_CONSISTENCY;
+    const bool nodupes = consistency ? cfg->no_dupes != 0 : true;  // cpu.cpp:70-77
+    const int max_lr = consistency ? cfg->max_lr_diff : -1;
+    const bool fallback = opts->fallback != 0 && thr;
+    const size_t npx = (size_t)W * H;
+    const size_t desc_bytes = align_up(npx * words * 4, 256);
+    const size_t raw_bytes = subpix ? align_up(npx * 2, 256) : 0;
+    const size_t cell_bytes = align_up(g.cells * 4, 256) + align_up(g.cells * 2, 256);
+    const size_t fb_bytes = fallback ? 2 * align_up(npx * 2, 256) + align_up((size_t)H * 4, 256) +
+                                           align_up(npx * 4, 256) + 256 +
+                                           (consistency ? 2 * align_up(npx * 2, 256) : 0)
+                                     : 0;
+    const size_t cnn_bytes = align_up(windows_scratch_floats(g) * 4, 256);
+    const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
+
+    int dev;
+    if ((rc = current_device(&dev))) return rc;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    void* ws;
+    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + cell_bytes + fb_bytes + cnn_bytes + pk0 + pk1, s, &ws)))
+        return rc;
+    uint8_t* p = (uint8_t*)ws;
+    uint32_t* desc0 = (uint32_t*)p; p += desc_bytes;
+    uint32_t* desc1 = (uint32_t*)p; p += desc_bytes;
+    int16_t* raw = subpix ? (int16_t*)p : nullptr; p += raw_bytes;
+    float* center = (float*)p; p += align_up(g.cells * 4, 256);
+    int16_t* hw = (int16_t*)p; p += align_up(g.cells * 2, 256);
+    int16_t *list = nullptr, *fwd = nullptr, *rev = nullptr;
+    int* row_count = nullptr;
+    uint32_t* glist = nullptr;
+    unsigned int* gcount = nullptr;
+    if (fallback) {
+        list = (int16_t*)p; p += align_up(npx * 2, 256);
+        p += align_up(npx * 2, 256);  // spare
+        row_count = (int*)p; p += align_up((size_t)H * 4, 256);
+        glist = (uint32_t*)p; p += align_up(npx * 4, 256);
+        gcount = (unsigned int*)p; p += 256;
+        if (consistency) {
+            fwd = (int16_t*)p; p += align_up(npx * 2, 256);
+            rev = (int16_t*)p; p += align_up(npx * 2, 256);
+        }
+    }
+    float* cnn_scratch = (float*)p; p += cnn_bytes;
+    uint8_t* pack0 = p;
+    uint8_t* pack1 = p + pk0;
+
+    auto body = [&]() -> int {
+        int r;
+        Stack v0, v1;
+        if ((r = make_stack(f0, n, i0, pack0, &v0, s))) return r;
+        if ((r = make_stack(f1, n, i1, pack1, &v1, s))) return r;
+        if ((r = transform_any(v0, &v1, i0.depth, n, cfg->mode, W, H, words, desc0, desc1, s))) return r;
+        if ((r = cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch, center, hw,
+                             nullptr, s)))
+            return r;
+        int16_t* d16 = subpix ? raw : (int16_t*)disparity->data;
+        const int64_t d16_step = subpix ? W : disparity->step / 2;
+        if ((r = run_window_search(desc0, desc1, g, words, nodupes, consistency, max_lr, center, hw,
+                                   d16, d16_step, s)))
+            return r;
+        if (!thr) return BICOS_OK;
+
+        const float mv_n = cfg->has_min_variance ? cfg->min_variance * (float)n : 0.0f;
+        void* cm = corrmap ? corrmap->data : nullptr;
+        const int64_t cstep = cm ? corrmap->step / elem_size(corrmap->type) : 0;
+        float* outf = subpix ? (float*)disparity->data : nullptr;
+        const int64_t ostep = subpix ? disparity->step / 4 : 0;
+        if (subpix)
+            r = run_subpixel(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                             cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision, outf,
+                             ostep, cm, cstep, s);
+        else
+            r = run_agree(d16, d16_step, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                          cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, s);
+        if (r || !fallback) return r;
+
+        // fallback: full-row search and NCC stage of the rejected pixels
+        if ((r = run_fallback_list(outf, subpix ? nullptr : d16, subpix ? ostep : d16_step, v0,
+                                   i0.depth, n, g, cfg->has_min_variance, mv_n, cfg->precision,
+                                   list, row_count, glist, gcount, s)))
+            return r;
+        if (consistency) {
+            if ((r = run_search_list(desc0, desc1, H, W, words, nodupes, SEARCH_REV, list, row_count,
+                                     fwd, nullptr, 0, s)))
+                return r;
+            if ((r = run_search_list(desc1, desc0, H, W, words, nodupes, SEARCH_REV, fwd, row_count,
+                                     rev, nullptr, 0, s)))
+                return r;
+            LAUNCH("lr_combine", k_lr_combine, dim3((W + 255) / 256, H), dim3(256), 0, s,
+                   (const int16_t*)list, (const int*)row_count, W, (const int16_t*)fwd,
+                   (const int16_t*)rev, max_lr, d16, d16_step);
+        } else {
+            if ((r = run_search_list(desc0, desc1, H, W, words, nodupes, SEARCH_DISP, list, row_count,
+                                     nullptr, d16, d16_step, s)))
+                return r;
+        }
+        const int blocks = (int)std::min<size_t>((npx + 255) / 256, 2048);
+        if (subpix)
+            return run_subpixel_list(raw, W, v0, v1, i0.depth, n, W, cfg->nxcorr_threshold,
+                                     cfg->subpixel_step, cfg->has_min_variance, mv_n,
+                                     cfg->precision, outf, ostep, cm, cstep, glist, gcount, blocks, s);
+        return run_agree_list(d16, d16_step, v0, v1, i0.depth, n, W, cfg->nxcorr_threshold,
+                              cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, glist, gcount,
+                              blocks, s);
+    };
+    rc = body();
+    unsigned int fb_count = 0;
+    if (!rc && stats && fallback) {
+        const hipError_t e1 = hipMemcpyAsync(&fb_count, gcount, 4, hipMemcpyDeviceToHost, s);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(s) : e1;
+        if (e2 != hipSuccess) rc = fail(BICOS_ERR_HIP, std::string("fallback count: ") + hipGetErrorString(e2));
+    }
+    const int rc2 = ws_release(dev, s);
+    if (!rc && stats) {
+        stats->pixels = (int64_t)npx;
+        stats->fallback_pixels = fb_count;
+    }
+    return rc ? rc : rc2;
+}
+
+}  // extern "C"

@@ -1,0 +1,285 @@
+// Windowed Hamming search and full-search fallback of the learned-window
+// matching path (no reference counterpart; the search semantics restricted
+// to a window are those of include/impl/cpu/bicos.hpp:50-113).
+//
+// Every pixel searches the candidate columns [c - hw, c + hw] (clipped to the
+// row) around c = col - round(centre) of its 4 x 4 cell's window. Keys and
+// the duplicate rule are the full search's (search.hpp), restricted to the
+// window. Pixels whose windowed match is then rejected by the NCC stage are
+// listed (k_fallback_list) and searched over the whole row
+// (k_search_list), and their NCC stage is run again on the list.
+#pragma once
+
+#include "agree.hpp"
+#include "cnn.hpp"
+#include "device_common.hpp"
+#include "search.hpp"
+
+namespace bicos_hip {
+
+template<int WORDS>
+__device__ __forceinline__ void load_desc(uint32_t (&q)[WORDS], const uint32_t* p) {
+    if constexpr (WORDS >= 4) {
+#pragma unroll
+        for (int k = 0; k < WORDS; k += 4) {
+            const uint4 v = *(const uint4*)(p + k);
+            q[k] = v.x; q[k + 1] = v.y; q[k + 2] = v.z; q[k + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < WORDS; ++k) q[k] = p[k];
+    }
+}
+
+// two smallest keys of the candidates [lo, hi] of row `crow` against q
+template<int WORDS, bool NODUPES>
+__device__ __forceinline__ void scan_window(const uint32_t (&q)[WORDS], const uint32_t* crow, int lo,
+                                            int hi, uint32_t& b1, uint32_t& b2) {
+    b1 = 0xFFFFFFFFu;
+    b2 = 0xFFFFFFFFu;
+    for (int c = lo; c <= hi; ++c) {
+        uint32_t d[WORDS];
+        load_desc<WORDS>(d, crow + (size_t)c * WORDS);
+        uint32_t cost = 0;
+#pragma unroll
+        for (int k = 0; k < WORDS; ++k) cost = __popc(q[k] ^ d[k]) + cost;
+        const uint32_t key = (cost << 16) | (uint32_t)c;
+        if constexpr (NODUPES) b2 = umed3(b1, b2, key);
+        b1 = min(b1, key);
+    }
+}
+
+// int16 disparity from the windowed search. CONSIST: the matched right
+// column searches the left row back over [col - hw, col + hw] and the match
+// stands if that reverse best lies within max_lr_diff of col
+// (bicos.hpp:96-106 restricted to the window).
+template<int WORDS, bool NODUPES, bool CONSIST>
+__global__ __launch_bounds__(256) void k_window_search(const uint32_t* __restrict__ d0,
+                                                       const uint32_t* __restrict__ d1, int W,
+                                                       const float* __restrict__ center_q,
+                                                       const int16_t* __restrict__ hw_q, int Hq,
+                                                       int Wq, int max_lr_diff,
+                                                       int16_t* __restrict__ out, int64_t ostep) {
+    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
+    if (col >= W)
+        return;
+    const int cq = min(row / CNN_DS, Hq - 1) * Wq + min(col / CNN_DS, Wq - 1);
+    const int c1 = col - (int)__builtin_rintf(center_q[cq]);
+    const int hw = hw_q[cq];
+    const int lo = max(c1 - hw, 0), hi = min(c1 + hw, W - 1);
+    uint32_t q[WORDS];
+    load_desc<WORDS>(q, d0 + ((size_t)row * W + col) * WORDS);
+    uint32_t b1, b2;
+    scan_window<WORDS, NODUPES>(q, d1 + (size_t)row * W * WORDS, lo, hi, b1, b2);
+    int16_t v = INVALID16;
+    const bool valid = lo <= hi && !(NODUPES && (b1 >> 16) == (b2 >> 16));
+    if (valid) {
+        const int best = (int)(b1 & 0xFFFFu);
+        if constexpr (!CONSIST) {
+            v = (int16_t)(col - best);
+        } else {
+            uint32_t r[WORDS];
+            load_desc<WORDS>(r, d1 + ((size_t)row * W + best) * WORDS);
+            uint32_t r1, r2;
+            scan_window<WORDS, NODUPES>(r, d0 + (size_t)row * W * WORDS, max(col - hw, 0),
+                                        min(col + hw, W - 1), r1, r2);
+            const int r0 = (int)(r1 & 0xFFFFu);
+            if (!(NODUPES && (r1 >> 16) == (r2 >> 16)) && abs(col - r0) <= max_lr_diff)
+                v = (int16_t)((col + r0) / 2 - best);
+        }
+    }
+    out[(size_t)row * ostep + col] = v;
+}
+
+// Fallback set: pixels whose windowed result is invalid and whose left
+// variance does not already reject every match (has_minvar and
+// var0 < minvar, computed as the NCC stage does in precision TPrec). Writes
+// the per-row lists (columns ascending, row_count[row] entries at
+// list + row * W) and the packed (row << 16 | col) entries of the whole set
+// at glist[0 .. *gcount).
+template<typename TIn, typename TPrec>
+__global__ __launch_bounds__(256) void k_fallback_list(const float* __restrict__ disp_f,
+                                                       const int16_t* __restrict__ disp_i,
+                                                       int64_t dstep, const Stack f0, int n, int W,
+                                                       int has_minvar, float minvar,
+                                                       int16_t* __restrict__ list,
+                                                       int* __restrict__ row_count,
+                                                       uint32_t* __restrict__ glist,
+                                                       unsigned int* __restrict__ gcount) {
+    __shared__ int base, gbase;
+    __shared__ int wsum[4];
+    const int row = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < W; c0 += 256) {
+        const int col = c0 + threadIdx.x;
+        bool need = false;
+        if (col < W) {
+            const size_t i = (size_t)row * dstep + col;
+            need = disp_f ? __builtin_isnan(disp_f[i]) : disp_i[i] == INVALID16;
+            if (need && has_minvar) {
+                TPrec mean = 0;
+                for (int t = 0; t < n; ++t) mean += (TPrec)load_sample_cached<TIn>(f0, t, row, col);
+                mean /= (TPrec)n;
+                TPrec var = 0;
+                for (int t = 0; t < n; ++t) {
+                    const TPrec d = (TPrec)load_sample_cached<TIn>(f0, t, row, col) - mean;
+                    var = pfma(d, d, var);
+                }
+                need = !(var < (TPrec)minvar);
+            }
+        }
+        const uint64_t m = __ballot(need);
+        const int before = __popcll(m & ((1ull << lane) - 1));
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (need) list[(size_t)row * W + off + before] = (int16_t)col;
+        __syncthreads();
+        if (threadIdx.x == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        row_count[row] = base;
+        gbase = (int)atomicAdd(gcount, (unsigned int)base);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < base; i += 256)
+        glist[gbase + i] = ((uint32_t)row << 16) | (uint32_t)(uint16_t)list[(size_t)row * W + i];
+}
+
+// Full-row search of the listed query columns of each row (the row's
+// candidates staged in LDS as in k_search). Entry i of row r queries column
+// qcol[r * W + i] (skipped when negative) and writes
+//   MODE SEARCH_REV:  the best candidate column (INVALID16 if none/duplicate)
+//                     to res[r * W + i]
+//   MODE SEARCH_DISP: col - best to disp[r * dstep + col]
+// One block serves 256 entries of one row; waves past the row's count skip
+// the arithmetic but take part in the staging.
+template<int WORDS, bool NODUPES, int MODE>
+__global__ __launch_bounds__(SEARCH_THREADS) void k_search_list(
+    const uint32_t* __restrict__ dq, const uint32_t* __restrict__ dc, int W, int chunk_cols,
+    const int16_t* __restrict__ qcol, const int* __restrict__ row_count,
+    int16_t* __restrict__ res, int16_t* __restrict__ disp, int64_t dstep) {
+    extern __shared__ uint4 lds4[];
+    uint32_t* lds = (uint32_t*)lds4;
+    const int row = blockIdx.y, tid = threadIdx.x;
+    const int count = row_count[row];
+    const int i0 = blockIdx.x * SEARCH_THREADS;
+    if (i0 >= count)
+        return;  // uniform over the block
+    const int i = i0 + tid;
+    const int col = i < count ? qcol[(size_t)row * W + i] : -1;
+    const bool wave_busy = i0 + (tid & ~63) < count;
+    uint32_t q[WORDS];
+    load_desc<WORDS>(q, dq + ((size_t)row * W + (col >= 0 ? col : 0)) * WORDS);
+    uint32_t b1 = 0xFFFFFFFFu, b2 = 0xFFFFFFFFu;
+    const uint32_t* crow = dc + (size_t)row * W * WORDS;
+    for (int c0 = 0; c0 < W; c0 += chunk_cols) {
+        const int cn = min(chunk_cols, W - c0);
+        __syncthreads();
+        if constexpr (WORDS >= 4) {
+            const uint4* src = (const uint4*)(crow + (size_t)c0 * WORDS);
+            for (int k = tid; k < cn * (WORDS / 4); k += SEARCH_THREADS) lds4[k] = src[k];
+        } else {
+            const uint32_t* src = crow + (size_t)c0 * WORDS;
+            for (int k = tid; k < cn * WORDS; k += SEARCH_THREADS) lds[k] = src[k];
+        }
+        __syncthreads();
+        if (wave_busy) {
+#pragma unroll 4
+            for (int c = 0; c < cn; ++c) {
+                uint32_t cost = 0;
+#pragma unroll
+                for (int k = 0; k < WORDS; ++k) cost = __popc(q[k] ^ lds[c * WORDS + k]) + cost;
+                const uint32_t key = (cost << 16) | (uint32_t)(c0 + c);
+                if constexpr (NODUPES) b2 = umed3(b1, b2, key);
+                b1 = min(b1, key);
+            }
+        }
+    }
+    if (i >= count)
+        return;
+    const bool valid = col >= 0 && !(NODUPES && (b1 >> 16) == (b2 >> 16));
+    const int best = (int)(b1 & 0xFFFFu);
+    if constexpr (MODE == SEARCH_REV) {
+        res[(size_t)row * W + i] = valid ? (int16_t)best : INVALID16;
+    } else {
+        if (col >= 0) disp[(size_t)row * dstep + col] = valid ? (int16_t)(col - best) : INVALID16;
+    }
+}
+
+// Consistency fallback: entry i of row r matched left column list[i] with
+// right column fwd[i]; rev[i] is that right column's best left column.
+__global__ __launch_bounds__(256) void k_lr_combine(const int16_t* __restrict__ list,
+                                                    const int* __restrict__ row_count, int W,
+                                                    const int16_t* __restrict__ fwd,
+                                                    const int16_t* __restrict__ rev, int max_lr_diff,
+                                                    int16_t* __restrict__ disp, int64_t dstep) {
+    const int row = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= row_count[row])
+        return;
+    const size_t e = (size_t)row * W + i;
+    const int col = list[e], best = fwd[e], r0 = rev[e];
+    int16_t v = INVALID16;
+    if (best != INVALID16 && r0 != INVALID16 && abs(col - r0) <= max_lr_diff)
+        v = (int16_t)((col + r0) / 2 - best);
+    disp[(size_t)row * dstep + col] = v;
+}
+
+// cell labels for training: mean of the valid disparities of the 4 x 4
+// block when at least min_valid are valid, else NaN
+__global__ __launch_bounds__(256) void k_window_label(const float* __restrict__ disp_f,
+                                                      const int16_t* __restrict__ disp_i,
+                                                      int64_t dstep, int Hq, int Wq, int min_valid,
+                                                      float* __restrict__ label) {
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    float s = 0.0f;
+    int cnt = 0;
+    for (int dy = 0; dy < CNN_DS; ++dy)
+        for (int dx = 0; dx < CNN_DS; ++dx) {
+            const size_t i = (size_t)(Y * CNN_DS + dy) * dstep + X * CNN_DS + dx;
+            float v;
+            if (disp_f) {
+                v = disp_f[i];
+                if (__builtin_isnan(v)) continue;
+            } else {
+                if (disp_i[i] == INVALID16) continue;
+                v = (float)disp_i[i];
+            }
+            s += v;
+            ++cnt;
+        }
+    label[(size_t)Y * Wq + X] = cnt >= min_valid && cnt > 0 ? s / (float)cnt : __builtin_nanf("");
+}
+
+// sum of the per-cell losses and label counts, added to acc[0], acc[1]
+__global__ __launch_bounds__(256) void k_loss_sum(const float* __restrict__ cell_loss, size_t cells,
+                                                  float* __restrict__ acc) {
+    __shared__ double sl[256], sc[256];
+    double l = 0.0, c = 0.0;
+    for (size_t i = threadIdx.x; i < cells; i += 256) {
+        l += cell_loss[i];
+        c += cell_loss[cells + i];
+    }
+    sl[threadIdx.x] = l;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sl[threadIdx.x] += sl[threadIdx.x + o];
+            sc[threadIdx.x] += sc[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc[0] += (float)sl[0];
+        acc[1] += (float)sc[0];
+    }
+}
+
+}  // namespace bicos_hip

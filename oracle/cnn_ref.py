@@ -96,10 +96,10 @@ def features(x0, L):
     return acts
 
 
-def corr(fl, fr):
+def corr(fl, fr, bins=K):
     Cn, Hq, Wq = fl.shape
-    out = np.full((K, Hq, Wq), -1e30)
-    for k in range(K):
+    out = np.full((bins, Hq, Wq), -1e30)
+    for k in range(bins):
         d = D0 + k
         for X in range(Wq):
             xr = X - d
@@ -117,7 +117,7 @@ def softmax_ce(Cv, label):
     lz = mx + np.log(e.sum(0))
     p = np.where(valid, np.exp(Cv - lz), 0.0)
     pos = label / DS - D0
-    ok = np.isfinite(label) & (pos >= 0) & (pos <= K - 1)
+    ok = np.isfinite(label) & (pos >= 0) & (pos <= Kn - 1)
     k0 = np.floor(np.where(ok, pos, 0)).astype(int)
     w1 = np.where(ok, pos - k0, 0.0)
     # a cell whose target bins fall outside the right image is unlabelled
@@ -140,7 +140,7 @@ def corr_bwd(fl, fr, dC):
     dfl = np.zeros_like(fl)
     dfr = np.zeros_like(fr)
     s = 1 / np.sqrt(C)
-    for k in range(K):
+    for k in range(dC.shape[0]):
         d = D0 + k
         for X in range(Wq):
             xr = X - d
@@ -151,13 +151,13 @@ def corr_bwd(fl, fr, dC):
     return dfl, dfr
 
 
-def loss_and_grad(framesL, framesR, label, p, in_scale, bf=True):
+def loss_and_grad(framesL, framesR, label, p, in_scale, bf=True, bins=K):
     """mean cross-entropy over labelled cells and its gradient w.r.t. the flat
     parameters (weights through their BF16 values when bf)"""
     L = layers(p, bf)
     aL = features(cnn_input(framesL, in_scale), L)
     aR = features(cnn_input(framesR, in_scale), L)
-    Cv = corr(aL[-1], aR[-1])
+    Cv = corr(aL[-1], aR[-1], bins)
     loss, cnt, dC = softmax_ce(Cv, label)
     cnt = max(cnt, 1)
     dC /= cnt

@@ -134,3 +134,44 @@ def test_scene_generator_is_deterministic():
     assert 8.0 <= gt.min() and gt.max() <= 72.0
     u = scene.generate(scene.C1_1920x1200_U16, seed=5, height=8, width=300)[0]
     assert u.dtype == np.uint16 and u.max() > 40000
+
+
+def test_window_host_side():
+    """Learned-window entry points that run on the host: defaults, the
+    deterministic initial parameters and argument checks before device work."""
+    import libbicos_amd as B
+    from libbicos_amd import window as Wn
+    import cnn_ref
+    assert Wn.PARAM_COUNT == cnn_ref.PARAMS == 28928
+    o = B._WindowOptions()
+    B.lib().bicos_hip_window_options_default(C.byref(o))
+    py = Wn.WindowOptions()
+    assert (o.in_scale, o.p_conf, o.hw_lo, o.hw_hi, o.bins, o.fallback) == \
+        (py.in_scale, py.p_conf, py.hw_lo, py.hw_hi, py.bins, int(py.fallback))
+    a, b, c = Wn.init_params(3), Wn.init_params(3), Wn.init_params(4)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    for l in range(4):  # He-normal weights, zero biases
+        w = a[cnn_ref.w_off(l):cnn_ref.b_off(l)]
+        fan_in = (4 if l == 0 else 32) * 9
+        assert abs(w.std() * np.sqrt(fan_in / 2) - 1) < 0.1
+        assert not a[cnn_ref.b_off(l):cnn_ref.b_off(l) + 32].any()
+    s = _fake_stack(3)
+    cfg = B.Config()._c()
+    d = B._Image(None, 4, 8, 3, 0, 16)
+    rc = B.lib().bicos_hip_match_windowed(s, s, 3, C.byref(cfg), C.c_void_p(1), C.byref(o),
+                                          C.byref(d), None, None, None)
+    assert rc == B.ERR_INVALID_ARGUMENT and "4 frames" in B.lib().bicos_hip_last_error().decode()
+    o.bins = 3
+    s = _fake_stack(4)
+    rc = B.lib().bicos_hip_match_windowed(s, s, 4, C.byref(cfg), C.c_void_p(1), C.byref(o),
+                                          C.byref(d), None, None, None)
+    assert rc == B.ERR_INVALID_ARGUMENT and "bins" in B.lib().bicos_hip_last_error().decode()
+
+
+def test_window_weights_roundtrip(tmp_path):
+    from libbicos_amd import window as Wn
+    p = Wn.init_params(9)
+    f = str(tmp_path / "w.npz")
+    Wn.save(f, p, {"scenes": 12, "note": "x"})
+    q, meta = Wn.load(f)
+    assert np.array_equal(p, q) and meta == {"scenes": "12", "note": "'x'"}

@@ -1,0 +1,308 @@
+"""Learned search window on the GPU, through the C ABI.
+
+- the device scene generator against the host generator;
+- the CNN's loss and gradient (HIP forward/backward kernels) against the
+  numpy reference oracle/cnn_ref.py, itself pinned by finite differences;
+- Adam against its formula, labels against their definition;
+- the windowed matching path: bit-exact to the plain path when the window
+  spans the whole row, and the exact fallback invariant (a pixel the
+  learned path rejects is rejected by the plain path) for any network;
+- the trained network (committed weights): agreement with the plain path on
+  unseen 1920x1200x33 scenes and its gain over an untrained network.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C1_CFG = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    if t.dtype == torch.uint16:
+        return t.view(torch.int16).cpu().numpy().view(np.uint16)
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def W():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test run without a visible GPU")
+    from libbicos_amd import window
+    return window
+
+
+# ---------------------------------------------------------------- scenes
+
+@pytest.mark.parametrize("which", ["c1", "c0"])
+def test_scene_device_matches_host(W, which):
+    from libbicos_amd import scene
+    spec = scene.C1_1920x1200_U16 if which == "c1" else scene.C0_640x480_U8
+    kw = dict(height=40, width=200, frames=5)
+    L, R, D = scene.generate(spec, seed=11, threads=4, **kw)
+    dL, dR, dD = W.scene_device(spec, 11, frames=kw["frames"], height=kw["height"], width=kw["width"])
+    gL, gR, gD = host(dL), host(dR), host(dD)
+    # same construction and hashes; device and host libm may round
+    # sin/cos/log differently in the last place
+    assert np.abs(gD - D).max() <= 1e-3
+    for a, b in ((gL, L), (gR, R)):
+        diff = np.abs(a.astype(np.int64) - b.astype(np.int64))
+        assert (diff == 0).mean() >= 0.999 and diff.max() <= 1
+
+
+def test_scene_frames_independent_of_count(W):
+    from libbicos_amd import scene
+    spec = scene.C1_1920x1200_U16
+    a = W.scene_device(spec, 5, frames=6, height=32, width=96)
+    b = W.scene_device(spec, 5, frames=4, height=32, width=96)
+    assert np.array_equal(host(a[0])[:4], host(b[0]))
+    assert np.array_equal(host(a[1])[:4], host(b[1]))
+
+
+# ---------------------------------------------------------------- labels
+
+@pytest.mark.parametrize("kind", ["f32", "i16"])
+def test_labels(W, kind):
+    import torch
+    rng = np.random.default_rng(3)
+    H, Wd = 37, 50
+    d = rng.uniform(0, 100, size=(H, Wd)).astype(np.float32)
+    d[rng.random((H, Wd)) < 0.4] = np.nan
+    if kind == "i16":
+        di = np.where(np.isnan(d), -32768, np.round(d)).astype(np.int16)
+        t = torch.from_numpy(di).cuda()
+        ref_in = np.where(di == -32768, np.nan, di.astype(np.float32))
+    else:
+        t = torch.from_numpy(d).cuda()
+        ref_in = d
+    got = host(W.label(t, min_valid=8))
+    Hq, Wq = H // 4, Wd // 4
+    blk = ref_in[:Hq * 4, :Wq * 4].reshape(Hq, 4, Wq, 4)
+    cnt = np.isfinite(blk).sum(axis=(1, 3))
+    s = np.nansum(blk, axis=(1, 3))
+    exp = np.where(cnt >= 8, s / np.maximum(cnt, 1), np.nan).astype(np.float32)
+    assert np.allclose(got, exp, rtol=1e-6, atol=1e-4, equal_nan=True)
+
+
+# ---------------------------------------------------------- loss/gradient
+
+def small_scene(W, seed, H=48, Wd=160):
+    from libbicos_amd import scene
+    spec = scene.C1_1920x1200_U16
+    L, R, D = W.scene_device(spec, seed, frames=4, height=H, width=Wd)
+    Hq, Wq = H // 4, Wd // 4
+    gt = host(D)[:Hq * 4, :Wq * 4].reshape(Hq, 4, Wq, 4).mean(axis=(1, 3)).astype(np.float32)
+    gt[0, :5] = np.nan           # some unlabelled cells
+    return L, R, gt
+
+
+@pytest.mark.parametrize("bins", [80, 30])
+def test_loss_grad_matches_reference(W, bins):
+    import torch
+    import cnn_ref
+    L, R, gt = small_scene(W, 21)
+    gt = np.minimum(gt, 4 * (bins - 9))   # keep most labels inside the bins
+    p = W.init_params(4)
+    p[cnn_ref.b_off(0):cnn_ref.b_off(0) + 32] = np.random.default_rng(1).normal(0, 0.1, 32)
+    in_scale = 8 / 65535
+    loss_ref, g_ref, _ = cnn_ref.loss_and_grad(host(L), host(R), gt, p.astype(np.float64),
+                                               in_scale, bf=True, bins=bins)
+    params = W.params_device(p)
+    grad = torch.zeros_like(params)
+    loss = torch.zeros(2, dtype=torch.float32, device="cuda")
+    opts = W.WindowOptions(in_scale=in_scale, bins=bins)
+    lab = torch.from_numpy(gt).cuda()
+    W.loss_grad(L, R, params, lab, grad, loss, opts)
+    l, cnt = host(loss)
+    g = host(grad) / max(cnt, 1)
+    assert cnt > 0
+    assert abs(l / cnt - loss_ref) <= 1e-4 * max(1.0, abs(loss_ref)), (l / cnt, loss_ref)
+    scale = np.abs(g_ref).max()
+    err = np.abs(g - g_ref)
+    assert err.max() <= 2e-3 * scale, (err.max(), scale)
+    assert np.linalg.norm(g - g_ref) <= 1e-3 * np.linalg.norm(g_ref)
+    # accumulation: a second call adds the same again
+    W.loss_grad(L, R, params, lab, grad, loss, opts)
+    assert np.allclose(host(grad), 2 * g * cnt, rtol=1e-5, atol=1e-6 * scale * cnt)
+
+
+def test_adam_matches_formula(W):
+    import torch
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=W.PARAM_COUNT).astype(np.float32)
+    m = rng.normal(size=W.PARAM_COUNT).astype(np.float32) * 0.1
+    v = rng.random(W.PARAM_COUNT).astype(np.float32) * 0.01
+    g = rng.normal(size=W.PARAM_COUNT).astype(np.float32)
+    cnt = np.array([4.0], np.float32)
+    tp, tm, tv = (torch.from_numpy(a.copy()).cuda() for a in (p, m, v))
+    W.adam(tp, tm, tv, torch.from_numpy(g).cuda(), torch.from_numpy(cnt).cuda(), 1e-3, 3)
+    b1, b2 = np.float64(np.float32(0.9)), np.float64(np.float32(0.999))
+    gs = g.astype(np.float64) / 4.0
+    m2 = b1 * m + (1 - b1) * gs
+    v2 = b2 * v + (1 - b2) * gs * gs
+    p2 = p - 1e-3 * (m2 / (1 - b1 ** 3)) / (np.sqrt(v2 / (1 - b2 ** 3)) + 1e-8)
+    assert np.allclose(host(tm), m2, rtol=1e-5, atol=1e-7)
+    assert np.allclose(host(tv), v2, rtol=1e-5, atol=1e-9)
+    assert np.allclose(host(tp), p2, rtol=1e-5, atol=1e-6)
+
+
+def test_training_reduces_loss(W):
+    from libbicos_amd import scene
+    spec = scene.C1_1920x1200_U16
+    import torch
+    scenes = []
+    for i in range(6):
+        L, R, D = W.scene_device(spec, 900 + i, height=96, width=384)
+        d, _ = __import__("libbicos_amd").match_device(L, R, __import__("libbicos_amd").Config(**C1_CFG))
+        scenes.append((L[:4].contiguous(), R[:4].contiguous(), W.label(d)))
+    params = W.params_device(W.init_params(0))
+    m, v = torch.zeros_like(params), torch.zeros_like(params)
+    grad = torch.zeros_like(params)
+    loss = torch.zeros(2, dtype=torch.float32, device="cuda")
+    opts = W.WindowOptions()
+
+    def eval_loss():
+        loss.zero_()
+        g = torch.zeros_like(params)
+        for L, R, lab in scenes:
+            W.loss_grad(L, R, params, lab, g, loss, opts)
+        l, c = host(loss)
+        return l / c
+
+    first = eval_loss()
+    for step in range(1, 41):
+        grad.zero_()
+        loss.zero_()
+        for L, R, lab in scenes[:4]:
+            W.loss_grad(L, R, params, lab, grad, loss, opts)
+        W.adam(params, m, v, grad, loss[1:2], 3e-3, step)
+    last = eval_loss()
+    print(f"loss {first:.3f} -> {last:.3f}")
+    assert last < 0.8 * first
+
+
+# ------------------------------------------------------ windowed matching
+
+def c1_crop(W, seed, H=64, Wd=480, frames=33):
+    from libbicos_amd import scene
+    L, R, _ = W.scene_device(scene.C1_1920x1200_U16, seed, frames=frames, height=H, width=Wd)
+    return L, R
+
+
+@pytest.mark.parametrize("variant", ["nodup", "cons"])
+@pytest.mark.parametrize("subpix", [True, False])
+def test_full_width_window_equals_plain(W, variant, subpix):
+    """A window spanning the row makes the windowed search the full search:
+    the learned path must then reproduce the plain path bit for bit."""
+    import libbicos_amd as B
+    L, R = c1_crop(W, 31)
+    kw = dict(C1_CFG)
+    if not subpix:
+        kw.pop("subpixel_step")
+    if variant == "cons":
+        kw["variant"] = B.Variant.Consistency(max_lr_diff=1, no_dupes=False)
+    cfg = B.Config(**kw)
+    params = W.params_device(W.init_params(1))
+    opts = W.WindowOptions(hw_lo=4000, hw_hi=4000)
+    d_plain, c_plain = B.match_device(L, R, cfg, want_corrmap=True)
+    d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, want_corrmap=True, stats=True)
+    a, b = host(d_plain), host(d_win)
+    assert np.array_equal(a, b, equal_nan=True)
+    assert np.array_equal(host(c_plain), host(c_win), equal_nan=True)
+    assert st["pixels"] == a.size
+
+
+@pytest.mark.parametrize("variant", ["nodup", "cons"])
+@pytest.mark.parametrize("subpix", [True, False])
+@pytest.mark.parametrize("hw", [0, 3])
+def test_fallback_invariant_untrained(W, variant, subpix, hw):
+    """With any network, every pixel the learned path rejects was sent
+    through the exact full search and NCC stage: the plain path rejects it
+    too. Pixels the learned path accepts either match the plain result or
+    pass the NCC threshold at another disparity."""
+    import libbicos_amd as B
+    L, R = c1_crop(W, 32 + hw)
+    kw = dict(C1_CFG)
+    if not subpix:
+        kw.pop("subpixel_step")
+    if variant == "cons":
+        kw["variant"] = B.Variant.Consistency(max_lr_diff=1, no_dupes=False)
+    cfg = B.Config(**kw)
+    params = W.params_device(W.init_params(2))
+    opts = W.WindowOptions(hw_lo=hw, hw_hi=hw)
+    d_plain, _ = B.match_device(L, R, cfg)
+    d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, want_corrmap=True, stats=True)
+    a, b = host(d_plain), host(d_win)
+    inv_a = np.isnan(a) if subpix else a == -32768
+    inv_b = np.isnan(b) if subpix else b == -32768
+    assert not (inv_b & ~inv_a).any(), "learned path rejected a pixel the plain path accepts"
+    cw = host(c_win)
+    assert (cw[~inv_b] >= 0.96 - 1e-6).all()
+    assert st["fallback_pixels"] > 0
+    print(f"hw {hw}: fallback {st['fallback_pixels'] / a.size:.3f}, "
+          f"agree {(a == b)[~inv_a].mean():.4f}")
+
+
+def test_windowed_errors(W):
+    import torch
+    import libbicos_amd as B
+    L, R = c1_crop(W, 40, H=16, Wd=64)
+    params = W.params_device(W.init_params(0))
+    with pytest.raises(ValueError, match="4 frames"):
+        W.match_windowed_device(L[:3], R[:3], B.Config(), params)
+    with pytest.raises(B.Exception):
+        W.match_windowed_device(L, R, B.Config(), params[:10].contiguous())
+    with pytest.raises(ValueError):
+        W.match_windowed_device(L, R, B.Config(), params, W.WindowOptions(bins=4))
+    with pytest.raises(ValueError):
+        W.match_windowed_device(L[:, :3], R[:, :3], B.Config(), params)
+
+
+# ---------------------------------------------------------- trained weights
+
+def trained(W):
+    if not os.path.exists(W.DEFAULT_WEIGHTS):
+        pytest.skip("no trained window weights committed")
+    p, meta = W.load()
+    return W.params_device(p), meta
+
+
+def test_trained_windows_contain_labels(W):
+    """The gain of the trained network over the untrained one: the share of
+    labelled cells whose window contains the plain-path disparity."""
+    from libbicos_amd import scene
+    params, _ = trained(W)
+    L, R, lab = W.scene_labels(scene.C1_1920x1200_U16, W.EVAL_SEED0 + 7)
+    c, hw, _ = W.predict(L, R, params)
+    hit, err = W.window_hits(c, hw, lab)
+    c0, hw0, _ = W.predict(L, R, W.params_device(W.init_params(0)))
+    hit0, err0 = W.window_hits(c0, hw0, lab)
+    print(f"trained: hit {hit:.4f} mean err {err:.2f} px; untrained: hit {hit0:.4f} err {err0:.2f} px")
+    assert hit >= 0.97 and hit0 <= 0.5
+
+
+def test_trained_agreement_c1_full_size(W):
+    """1920x1200x33 u16, unseen scenes: >= 99.5 % of the plain-valid pixels
+    are valid in the learned path and within 0.5 px."""
+    import libbicos_amd as B
+    from libbicos_amd import scene
+    params, _ = trained(W)
+    cfg = B.Config(precision=B.Precision.HalfMFMA, **C1_CFG)
+    for k in range(2):
+        L, R, _ = W.scene_device(scene.C1_1920x1200_U16, W.EVAL_SEED0 + 100 + k)
+        a = host(B.match_device(L, R, B.Config(**C1_CFG))[0])
+        d, _, st = W.match_windowed_device(L, R, cfg, params, stats=True)
+        b = host(d)
+        va = np.isfinite(a)
+        ok = np.isfinite(b) & (np.abs(b - a) <= 0.5)
+        agree = ok[va].mean()
+        extra = (np.isfinite(b) & ~va).mean()
+        fb = st["fallback_pixels"] / st["pixels"]
+        print(f"scene {k}: agreement {agree:.5f}, learned-valid/plain-invalid {extra:.5f}, "
+              f"fallback {fb:.4f}")
+        assert agree >= 0.995
