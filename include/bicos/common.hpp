@@ -5,7 +5,8 @@
  * OpenCV removed: `Image` is this library's own device image (the role
  * cv::cuda::GpuMat plays in the reference's CUDA build, common.hpp:50-56).
  * Precision gains HALF_MFMA (FP16 matrix-core NCC candidates with FP32
- * rescoring), which the reference does not have.
+ * rescoring) and MOMENTS (closed-form candidates from exact sample moments
+ * with FP32 rescoring), which the reference does not have.
  */
 #pragma once
 
@@ -71,7 +72,7 @@ private:
 };
 
 enum class TransformMode { LIMITED, FULL };
-enum class Precision { SINGLE, DOUBLE, HALF_MFMA };
+enum class Precision { SINGLE, DOUBLE, HALF_MFMA, MOMENTS };
 
 namespace Variant {
     struct NoDuplicates {};

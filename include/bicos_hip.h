@@ -53,10 +53,20 @@ enum {
 /* TransformMode (include/common.hpp:58) */
 enum { BICOS_MODE_LIMITED = 0, BICOS_MODE_FULL = 1 };
 
-/* Precision (include/common.hpp:59-61). BICOS_PRECISION_HALF_MFMA has no
- * reference counterpart: NCC candidates scored in FP16 on the matrix cores,
- * winners rescored in FP32 (not bit-exact; see the precision tests). */
-enum { BICOS_PRECISION_SINGLE = 0, BICOS_PRECISION_DOUBLE = 1, BICOS_PRECISION_HALF_MFMA = 2 };
+/* Precision (include/common.hpp:59-61). HALF_MFMA and MOMENTS have no
+ * reference counterpart and concern the subpixel stage (the single-point
+ * agreement stays the exact FP32 one); neither is bit-exact (see the
+ * precision tests):
+ *   HALF_MFMA  grid points scored from FP16 matrix-core cross products and
+ *              FP64 means/variances, winners rescored in FP32;
+ *   MOMENTS    grid points scored in closed form from exact (FP64) sample
+ *              moments, winners rescored in FP32. */
+enum {
+    BICOS_PRECISION_SINGLE = 0,
+    BICOS_PRECISION_DOUBLE = 1,
+    BICOS_PRECISION_HALF_MFMA = 2,
+    BICOS_PRECISION_MOMENTS = 3
+};
 
 /* SearchVariant (include/common.hpp:63-71) */
 enum { BICOS_VARIANT_NO_DUPLICATES = 0, BICOS_VARIANT_CONSISTENCY = 1 };
@@ -185,7 +195,10 @@ typedef struct bicos_window_options {
     int32_t hw_lo;         /* window half-width (px) of confident cells */
     int32_t hw_hi;         /* ... and of the others */
     int32_t bins;          /* correlation offsets: disparities 4 * (-8 .. bins - 9) px */
-    int32_t fallback;      /* 1: full-row search for pixels whose windowed match fails */
+    int32_t fallback;      /* 1: full-row search of the pixels whose windowed match the
+                              NCC stage rejects (those whose left variance rejects every
+                              match excepted); a pixel the windowed path then rejects is
+                              one the plain path rejects too. 0: no fallback */
 } bicos_window_options;
 
 typedef struct bicos_window_stats {
@@ -195,6 +208,14 @@ typedef struct bicos_window_stats {
 
 int bicos_hip_window_param_count(void);
 void bicos_hip_window_options_default(bicos_window_options* opts);
+
+/* Network implementation of predict/match_windowed for subsequent calls:
+ * BICOS_CNN_MFMA (default; cell-major BF16 tensors, 3x3x32 layers and the
+ * correlation on v_mfma_f32_32x32x16_bf16, bins <= 160) or BICOS_CNN_VALU
+ * (the training kernels). Both compute the same network up to FP32
+ * summation order. */
+enum { BICOS_CNN_MFMA = 0, BICOS_CNN_VALU = 1 };
+int bicos_hip_set_cnn_impl(int32_t impl);
 
 /* Deterministic He-normal initial parameters (biases 0), host memory. */
 int bicos_hip_window_init(float* host_params, uint64_t seed);
@@ -233,7 +254,7 @@ int bicos_hip_window_predict(const bicos_image* stack0, const bicos_image* stack
  * parameters and options. Pixels whose windowed match the NCC stage rejects
  * (and whose left variance does not reject every match) are searched over the
  * full row and their NCC stage is run again (exact FP32 for SINGLE and
- * HALF_MFMA, FP64 for DOUBLE). stats: NULL, or host memory filled after the
+ * HALF_MFMA, FP64 for DOUBLE, MOMENTS for MOMENTS). stats: NULL, or host memory filled after the
  * call completes (the call then waits for the stream). Images need at least
  * 4 rows and columns and fewer than 65536 rows. */
 int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,

@@ -99,6 +99,7 @@ def _load():
     L.bicos_hip_window_options_default.argtypes = [P(_WindowOptions)]
     L.bicos_hip_window_options_default.restype = None
     L.bicos_hip_window_init.argtypes = [vp, C.c_uint64]
+    L.bicos_hip_set_cnn_impl.argtypes = [i32]
     L.bicos_hip_window_label.argtypes = [P(_Image), i32, vp, vp]
     L.bicos_hip_window_loss_grad.argtypes = [P(_Image), P(_Image), u32, vp, P(_WindowOptions), vp,
                                              vp, vp, vp]
@@ -125,7 +126,7 @@ EXPORTED_SYMBOLS = (
     "bicos_hip_device_synchronize", "bicos_hip_window_param_count",
     "bicos_hip_window_options_default", "bicos_hip_window_init", "bicos_hip_window_label",
     "bicos_hip_window_loss_grad", "bicos_hip_window_adam", "bicos_hip_window_predict",
-    "bicos_hip_match_windowed", "bicos_hip_scene_generate",
+    "bicos_hip_match_windowed", "bicos_hip_scene_generate", "bicos_hip_set_cnn_impl",
 )
 
 
@@ -153,10 +154,13 @@ class TransformMode:
 
 
 class Precision:
-    """include/common.hpp:59-61; HalfMFMA has no reference counterpart."""
+    """include/common.hpp:59-61; HalfMFMA and Moments have no reference
+    counterpart (subpixel candidates scored from FP16 matrix-core products,
+    resp. in closed form from exact moments; winners rescored in FP32)."""
     Single = 0
     Double = 1
     HalfMFMA = 2
+    Moments = 3
 
 
 class Variant:

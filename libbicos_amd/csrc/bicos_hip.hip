@@ -22,6 +22,7 @@
 #include "../../include/bicos_hip.h"
 #include "agree.hpp"
 #include "agree_half.hpp"
+#include "agree_moments.hpp"
 #include "search.hpp"
 #include "search_mfma.hpp"
 #include "host_common.hpp"
@@ -94,6 +95,11 @@ int ws_release(int dev, hipStream_t s, int slot) {
 }
 
 /* ------------------------------------------------------------- validation */
+
+bool precision_known(int p) {
+    return p == BICOS_PRECISION_SINGLE || p == BICOS_PRECISION_DOUBLE ||
+           p == BICOS_PRECISION_HALF_MFMA || p == BICOS_PRECISION_MOMENTS;
+}
 
 int elem_size(int type) {
     switch (type) {
@@ -267,7 +273,7 @@ int run_search_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool
     constexpr size_t lds = (size_t)CH * (2 * WORDS + 1) * 16;
     static_assert(lds <= 64 * 1024, "LDS chunk too large");
     const dim3 grid((W + WAVES * NB * 32 - 1) / (WAVES * NB * 32), H), block(WAVES * 64);
-#define SM(ND, M) LAUNCH("search_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep)
+#define SM(ND, M) LAUNCH("search_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep, (const int16_t*)nullptr, (const int*)nullptr, (int16_t*)nullptr)
     if (nodupes) {
         if (mode == SEARCH_DISP) SM(true, SEARCH_DISP);
         else if (mode == SEARCH_REV) SM(true, SEARCH_REV);
@@ -385,10 +391,92 @@ int run_subpixel_half_t(const int16_t* disp, int64_t dstep, const Stack& f0,
     });
 }
 
+template<typename TIn>
+int run_subpixel_moments_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                           uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
+                           float* out, int64_t ostep, uint8_t* status, hipStream_t s) {
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        constexpr int NMAX = decltype(NM)::value, NEX = decltype(NE)::value;
+        LAUNCH("agree_subpixel_moments", (k_agree_subpixel_moments<TIn, NMAX, NEX>), dim3((W + 255) / 256, H), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, status);
+        return BICOS_OK;
+    });
+}
+
+template<typename TIn>
+int run_moments_list_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                       uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
+                       float* out, int64_t ostep, uint8_t* status, const int16_t* list,
+                       const int* row_count, hipStream_t s) {
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        LAUNCH("agree_subpixel_moments_list", (k_agree_subpixel_moments_list<TIn, decltype(NM)::value, decltype(NE)::value>), dim3(H, (W + 255) / 256), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, status, list, row_count);
+        return BICOS_OK;
+    });
+}
+
+template<typename TIn>
+int run_wide_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, uint32_t n,
+               int W, int H, float thr, float step, int has_mv, float mv, float* out, int64_t ostep,
+               float* corr, int64_t cstep, const int16_t* list, const int* row_count,
+               uint8_t* status, hipStream_t s) {
+    const int npts = subpixel_points(step);
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        LAUNCH("agree_subpixel_wide", (k_agree_subpixel_wide<TIn, decltype(NM)::value, decltype(NE)::value>), dim3(H, 8), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, npts, has_mv, mv, out, ostep, corr, cstep, list, row_count, status);
+        return BICOS_OK;
+    });
+}
+
+// the pending pixels of the MOMENTS status, through the exact stage
+int run_pending_exact(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                      int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                      float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
+                      hipStream_t s) {
+    int rc;
+    if ((rc = run_status_list(ms->status, nullptr, 0, ST_PENDING, W, H, ms->list, ms->row_count, s)))
+        return rc;
+    return depth == BICOS_8U
+        ? run_wide_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, nullptr, 0, ms->list, ms->row_count, ms->status, s)
+        : run_wide_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, nullptr, 0, ms->list, ms->row_count, ms->status, s);
+}
+
+int run_moments_listed(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                       int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                       float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
+                       const int16_t* list, const int* row_count, hipStream_t s) {
+    int rc = depth == BICOS_8U
+        ? run_moments_list_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, list, row_count, s)
+        : run_moments_list_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, list, row_count, s);
+    if (rc) return rc;
+    return run_pending_exact(disp, dstep, f0, f1, depth, n, W, H, thr, step, has_mv, mv_n, out,
+                             ostep, ms, s);
+}
+
+int run_status_list(const uint8_t* status, const float* res, int64_t rstep, int mask, int W, int H,
+                    int16_t* list, int* row_count, hipStream_t s) {
+    LAUNCH("status_list", k_zero_i32, dim3((H + 255) / 256), dim3(256), 0, s, row_count, H);
+    LAUNCH("status_list", k_status_list, dim3(H, (W + 255) / 256), dim3(256), 0, s, status, res,
+           rstep, mask, W, list, row_count);
+    return BICOS_OK;
+}
+
 int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                  int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
                  float mv_n, int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
-                 hipStream_t s) {
+                 hipStream_t s, const MomentsScratch* ms) {
+    if (precision == BICOS_PRECISION_MOMENTS && !corr) {
+        // closed-form decisions, then the exact stage on the pending pixels
+        if (!ms) return fail(BICOS_ERR_INVALID_ARGUMENT, "MOMENTS needs its scratch buffers");
+        int rc = depth == BICOS_8U
+            ? run_subpixel_moments_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s)
+            : run_subpixel_moments_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s);
+        if (rc) return rc;
+        return run_pending_exact(disp, dstep, f0, f1, depth, n, W, H, thr, step, has_mv, mv_n,
+                                 out, ostep, ms, s);
+    }
+    if (precision == BICOS_PRECISION_MOMENTS)  // a corrmap is written: the exact stage throughout
+        precision = BICOS_PRECISION_SINGLE;
     if (precision == BICOS_PRECISION_HALF_MFMA) {
         return depth == BICOS_8U
             ? run_subpixel_half_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s)
@@ -407,10 +495,10 @@ int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stac
 template<typename TIn, typename TPrec>
 int run_agree_list_t(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, uint32_t n,
                      int W, float thr, int has_mv, float mv, TPrec* corr, int64_t cstep,
-                     const uint32_t* list, const unsigned int* count, int blocks, hipStream_t s) {
+                     const int16_t* list, const int* row_count, int H, hipStream_t s) {
     return with_nmax(n, [&](auto NM, auto NE) -> int {
-        LAUNCH("agree_list", (k_agree_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(blocks), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, has_mv, mv, corr, cstep, list, count);
+        LAUNCH("agree_list", (k_agree_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(H, (W + 255) / 256), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, has_mv, mv, corr, cstep, list, row_count);
         return BICOS_OK;
     });
 }
@@ -418,39 +506,39 @@ int run_agree_list_t(int16_t* disp, int64_t dstep, const Stack& f0, const Stack&
 template<typename TIn, typename TPrec>
 int run_subpixel_list_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                         uint32_t n, int W, float thr, float step, int has_mv, float mv, float* out,
-                        int64_t ostep, TPrec* corr, int64_t cstep, const uint32_t* list,
-                        const unsigned int* count, int blocks, hipStream_t s) {
+                        int64_t ostep, TPrec* corr, int64_t cstep, const int16_t* list,
+                        const int* row_count, int H, hipStream_t s) {
     return with_nmax(n, [&](auto NM, auto NE) -> int {
-        LAUNCH("agree_subpixel_list", (k_agree_subpixel_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(blocks), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep, list, count);
+        LAUNCH("agree_subpixel_list", (k_agree_subpixel_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(H, (W + 255) / 256), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep, list, row_count);
         return BICOS_OK;
     });
 }
 
 int run_agree_list(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
                    uint32_t n, int W, float thr, int has_mv, float mv_n, int precision, void* corr,
-                   int64_t cstep, const uint32_t* list, const unsigned int* count, int blocks,
+                   int64_t cstep, const int16_t* list, const int* row_count, int H,
                    hipStream_t s) {
     if (precision == BICOS_PRECISION_DOUBLE)
         return depth == BICOS_8U
-            ? run_agree_list_t<uint8_t, double>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (double*)corr, cstep, list, count, blocks, s)
-            : run_agree_list_t<uint16_t, double>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (double*)corr, cstep, list, count, blocks, s);
+            ? run_agree_list_t<uint8_t, double>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (double*)corr, cstep, list, row_count, H, s)
+            : run_agree_list_t<uint16_t, double>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (double*)corr, cstep, list, row_count, H, s);
     return depth == BICOS_8U
-        ? run_agree_list_t<uint8_t, float>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (float*)corr, cstep, list, count, blocks, s)
-        : run_agree_list_t<uint16_t, float>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (float*)corr, cstep, list, count, blocks, s);
+        ? run_agree_list_t<uint8_t, float>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (float*)corr, cstep, list, row_count, H, s)
+        : run_agree_list_t<uint16_t, float>(disp, dstep, f0, f1, n, W, thr, has_mv, mv_n, (float*)corr, cstep, list, row_count, H, s);
 }
 
 int run_subpixel_list(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                       int depth, uint32_t n, int W, float thr, float step, int has_mv, float mv_n,
                       int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
-                      const uint32_t* list, const unsigned int* count, int blocks, hipStream_t s) {
+                      const int16_t* list, const int* row_count, int H, hipStream_t s) {
     if (precision == BICOS_PRECISION_DOUBLE)
         return depth == BICOS_8U
-            ? run_subpixel_list_t<uint8_t, double>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, list, count, blocks, s)
-            : run_subpixel_list_t<uint16_t, double>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, list, count, blocks, s);
+            ? run_subpixel_list_t<uint8_t, double>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, list, row_count, H, s)
+            : run_subpixel_list_t<uint16_t, double>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (double*)corr, cstep, list, row_count, H, s);
     return depth == BICOS_8U
-        ? run_subpixel_list_t<uint8_t, float>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, list, count, blocks, s)
-        : run_subpixel_list_t<uint16_t, float>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, list, count, blocks, s);
+        ? run_subpixel_list_t<uint8_t, float>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, list, row_count, H, s)
+        : run_subpixel_list_t<uint16_t, float>(disp, dstep, f0, f1, n, W, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, list, row_count, H, s);
 }
 
 // Number of points of the subpixel grid `for (float x = -1; x <= 1; x += step)`
@@ -517,18 +605,18 @@ int current_device(int* dev) {
 // non-uniformly spaced ones into the workspace.
 template<typename F>
 int with_stacks(const FrameTable* f, const StackInfo* info, int count, uint32_t n, hipStream_t s,
-                F&& fn) {
+                F&& fn, size_t extra = 0) {
     size_t pk[2] = {0, 0};
     for (int k = 0; k < count; ++k) pk[k] = align_up(packed_bytes(f[k], n, info[k]), 256);
     int dev, rc;
     if ((rc = current_device(&dev))) return rc;
     std::lock_guard<std::mutex> lk(g_ws_mu);
     void* ws = nullptr;
-    if ((rc = ws_acquire(dev, pk[0] + pk[1] + 256, s, &ws))) return rc;
+    if ((rc = ws_acquire(dev, pk[0] + pk[1] + extra + 256, s, &ws))) return rc;
     Stack v[2];
     for (int k = 0; k < count && !rc; ++k)
         rc = make_stack(f[k], n, info[k], (uint8_t*)ws + (k ? pk[0] : 0), &v[k], s);
-    if (!rc) rc = fn(v);
+    if (!rc) rc = fn(v, (uint8_t*)ws + pk[0] + pk[1]);
     int rc2 = ws_release(dev, s);
     return rc ? rc : rc2;
 }
@@ -584,8 +672,7 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown transform mode");
     if (cfg->variant != BICOS_VARIANT_NO_DUPLICATES && cfg->variant != BICOS_VARIANT_CONSISTENCY)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented");  // cuda.cu.in:51
-    if (cfg->precision != BICOS_PRECISION_SINGLE && cfg->precision != BICOS_PRECISION_DOUBLE &&
-        cfg->precision != BICOS_PRECISION_HALF_MFMA)
+    if (!precision_known(cfg->precision))
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
 
     const int bits = required_bits(n, cfg->mode);
@@ -608,12 +695,14 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     const size_t raw_bytes = subpix ? align_up(npx * 2, 256) : 0;
     const size_t rev_bytes = consistency ? align_up(npx * 2, 256) : 0;
     const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
+    const bool moments = subpix && cfg->precision == BICOS_PRECISION_MOMENTS;
+    const size_t mom_bytes = moments ? MomentsScratch::bytes(npx, H) : 0;
 
     int dev;
     if ((rc = current_device(&dev))) return rc;
     std::lock_guard<std::mutex> lk(g_ws_mu);
     void* ws;
-    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + rev_bytes + pk0 + pk1, s, &ws))) return rc;
+    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + rev_bytes + pk0 + pk1 + mom_bytes, s, &ws))) return rc;
     uint8_t* p = (uint8_t*)ws;
     uint32_t* desc0 = (uint32_t*)p;
     uint32_t* desc1 = (uint32_t*)(p + desc_bytes);
@@ -621,6 +710,7 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     int16_t* rev = consistency ? (int16_t*)(p + 2 * desc_bytes + raw_bytes) : nullptr;
     uint8_t* pack0 = p + 2 * desc_bytes + raw_bytes + rev_bytes;
     uint8_t* pack1 = pack0 + pk0;
+    const MomentsScratch ms = MomentsScratch::carve(pack1 + pk1, npx);
 
     auto body = [&]() -> int {
         int r;
@@ -646,7 +736,8 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
         if (subpix)
             return run_subpixel(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                                 cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision,
-                                (float*)disparity->data, disparity->step / 4, cm, cstep, s);
+                                (float*)disparity->data, disparity->step / 4, cm, cstep, s,
+                                moments ? &ms : nullptr);
         return run_agree(d16, d16_step, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                          cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, s);
     };
@@ -697,7 +788,7 @@ int bicos_hip_transform(const bicos_image* frames, uint32_t n, int32_t mode, int
         return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor too narrow for this stack");
     if (!out) return fail(BICOS_ERR_EXCEPTION, "null output");
     hipStream_t s = (hipStream_t)stream;
-    return with_stacks(&f, &info, 1, n, s, [&](const Stack* v) {
+    return with_stacks(&f, &info, 1, n, s, [&](const Stack* v, void*) {
         return transform_any(v[0], nullptr, info.depth, n, mode, info.cols, info.rows, words, out,
                              nullptr, s);
     });
@@ -733,12 +824,11 @@ int bicos_hip_agree(int16_t* disparity, const bicos_image* stack0, const bicos_i
     if ((rc = check_stack(stack0, n, &i[0], &f[0]))) return rc;
     if ((rc = check_stack(stack1, n, &i[1], &f[1]))) return rc;
     if (!disparity) return fail(BICOS_ERR_EXCEPTION, "null disparity");
-    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE &&
-        precision != BICOS_PRECISION_HALF_MFMA)
+    if (!precision_known(precision))
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
     const float mv_n = has_min_variance ? min_variance * (float)n : 0.0f;
     hipStream_t s = (hipStream_t)stream;
-    return with_stacks(f, i, 2, n, s, [&](const Stack* v) {
+    return with_stacks(f, i, 2, n, s, [&](const Stack* v, void*) {
         return run_agree(disparity, i[0].cols, v[0], v[1], i[0].depth, n, i[0].cols, i[0].rows,
                          thr, has_min_variance, mv_n, precision, corrmap, i[0].cols, s);
     });
@@ -755,16 +845,17 @@ int bicos_hip_agree_subpixel(const int16_t* disparity, const bicos_image* stack0
     if ((rc = check_stack(stack1, n, &i[1], &f[1]))) return rc;
     if (!disparity || !out) return fail(BICOS_ERR_EXCEPTION, "null buffer");
     if ((rc = check_step(subpixel_step))) return rc;
-    if (precision != BICOS_PRECISION_SINGLE && precision != BICOS_PRECISION_DOUBLE &&
-        precision != BICOS_PRECISION_HALF_MFMA)
+    if (!precision_known(precision))
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
     const float mv_n = has_min_variance ? min_variance * (float)n : 0.0f;
     hipStream_t s = (hipStream_t)stream;
-    return with_stacks(f, i, 2, n, s, [&](const Stack* v) {
+    const size_t npx = (size_t)i[0].rows * i[0].cols;
+    return with_stacks(f, i, 2, n, s, [&](const Stack* v, void* extra) {
+        const MomentsScratch ms = MomentsScratch::carve(extra, npx);
         return run_subpixel(disparity, i[0].cols, v[0], v[1], i[0].depth, n, i[0].cols, i[0].rows,
                             thr, subpixel_step, has_min_variance, mv_n, precision, out,
-                            i[0].cols, corrmap, i[0].cols, s);
-    });
+                            i[0].cols, corrmap, i[0].cols, s, &ms);
+    }, precision == BICOS_PRECISION_MOMENTS ? MomentsScratch::bytes(npx, i[0].rows) : 0);
 }
 
 int bicos_hip_timing_enable(int32_t enable) {

@@ -9,12 +9,13 @@
 // The window of a pixel is centred on 4 * (D0 + k*) with k* the refined
 // correlation peak of its cell. Training minimises the cross-entropy between
 // softmax_k C and the plain BICOS disparity of the cell (split linearly over
-// the two neighbouring bins). Weights are stored in BF16 for the forward
-// pass, accumulation is FP32; FP32 master weights and Adam moments live in
-// the caller's parameter buffers. Tensors are channel-planar [c][Y][X].
+// the two neighbouring bins). Weights and activations (x0, a1..a3, f) are
+// BF16 values, every product is accumulated in FP32; FP32 master weights and
+// Adam moments live in the caller's parameter buffers. The training kernels
+// keep channel-planar [c][Y][X] FP32 tensors (holding BF16 values); the
+// inference kernels keep cell-major [Y][X][c] BF16 tensors and run the
+// 3x3x32 layers and the correlation on the matrix cores.
 #pragma once
-
-#include <hip/hip_bf16.h>
 
 #include "device_common.hpp"
 
@@ -35,9 +36,12 @@ __host__ __device__ constexpr int cnn_w_off(int l) {
 __host__ __device__ constexpr int cnn_b_off(int l) { return cnn_w_off(l) + (l == 0 ? CNN_W1 : CNN_WL); }
 constexpr int CNN_PARAMS = cnn_w_off(4);
 
-__device__ __forceinline__ float bf16_round(float x) {
-    return __bfloat162float(__float2bfloat16(x));
-}
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// round to nearest even BF16 (v_cvt_pk_bf16_f32), as float / as bits
+__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
+__device__ __forceinline__ uint16_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 
 // x0 from the first CNN_IN frames of one view (strided stack)
 template<typename TIn>
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(256) void k_cnn_input(const Stack f, int W, int Hq,
     m *= 1.0f / CNN_IN;
     const size_t plane = (size_t)Hq * Wq;
 #pragma unroll
-    for (int t = 0; t < CNN_IN; ++t) x0[t * plane + (size_t)Y * Wq + X] = (q[t] - m) * in_scale;
+    for (int t = 0; t < CNN_IN; ++t) x0[t * plane + (size_t)Y * Wq + X] = bf16_round((q[t] - m) * in_scale);
 }
 
 // BF16 copy of the weights (biases stay FP32)
@@ -109,7 +113,7 @@ __global__ __launch_bounds__(256) void k_cnn_conv(const float* __restrict__ x, c
     }
 #pragma unroll
     for (int co = 0; co < CNN_C; ++co)
-        y[co * plane + (size_t)Y * Wq + X] = RELU ? fmaxf(acc[co], 0.0f) : acc[co];
+        y[co * plane + (size_t)Y * Wq + X] = bf16_round(RELU ? fmaxf(acc[co], 0.0f) : acc[co]);
 }
 
 // correlation volume C[k][Y][X] (masked bins: -1e30)
@@ -341,6 +345,210 @@ __global__ __launch_bounds__(256) void k_cnn_window(const float* __restrict__ C,
             if (den < 0.0f) kf += fminf(fmaxf(0.5f * (cm - cp) / den, -0.5f), 0.5f);
         }
     }
+    center[at] = CNN_DS * (CNN_D0 + kf);
+    halfwidth[at] = (int16_t)(pk >= p_conf ? hw_lo : hw_hi);
+    if (peak_prob) peak_prob[at] = pk;
+}
+
+/* ------------------------------------------------------------ inference */
+
+// Inference weights: layer 0 as in the flat layout (BF16-rounded floats),
+// layers 1..3 as BF16 bits in [layer-1][co][tap][ci] so that one 16-byte
+// load gives 8 consecutive input channels of one tap.
+constexpr int CNN_MFMA_W = 3 * CNN_C * 9 * CNN_C;
+
+__global__ void k_cnn_pack_mfma(const float* __restrict__ p, float* __restrict__ w0,
+                                uint16_t* __restrict__ wm) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < CNN_W1) w0[i] = bf16_round(p[cnn_w_off(0) + i]);
+    if (i < CNN_MFMA_W) {
+        const int l = i / (CNN_C * 9 * CNN_C), r = i % (CNN_C * 9 * CNN_C);
+        const int co = r / (9 * CNN_C), tap = (r / CNN_C) % 9, ci = r % CNN_C;
+        wm[i] = bf16_bits(p[cnn_w_off(l + 1) + (co * CNN_C + ci) * 9 + tap]);
+    }
+}
+
+// layer 0 (4 -> 32 channels, ReLU) on the vector ALU, from the planar x0
+// of view blockIdx.z to cell-major BF16
+__global__ __launch_bounds__(256) void k_cnn_conv0_cm(const float* __restrict__ x0a,
+                                                      const float* __restrict__ x0b,
+                                                      const float* __restrict__ w0,
+                                                      const float* __restrict__ b, int Hq, int Wq,
+                                                      uint16_t* __restrict__ ya,
+                                                      uint16_t* __restrict__ yb) {
+    __shared__ float ws[CNN_W1];
+    for (int i = threadIdx.x; i < CNN_W1; i += 256) ws[i] = w0[i];
+    __syncthreads();
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const float* x = blockIdx.z ? x0b : x0a;
+    uint16_t* y = blockIdx.z ? yb : ya;
+    const size_t plane = (size_t)Hq * Wq;
+    float v[CNN_IN][9];
+#pragma unroll
+    for (int ci = 0; ci < CNN_IN; ++ci)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int yy = Y + k / 3 - 1, xx = X + k % 3 - 1;
+            v[ci][k] = (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq) ? x[ci * plane + (size_t)yy * Wq + xx] : 0.0f;
+        }
+    uint32_t packed[CNN_C / 2];
+#pragma unroll
+    for (int co = 0; co < CNN_C; co += 2) {
+        float a0 = b[co], a1 = b[co + 1];
+#pragma unroll
+        for (int ci = 0; ci < CNN_IN; ++ci)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                a0 = __builtin_fmaf(ws[(co * CNN_IN + ci) * 9 + k], v[ci][k], a0);
+                a1 = __builtin_fmaf(ws[((co + 1) * CNN_IN + ci) * 9 + k], v[ci][k], a1);
+            }
+        packed[co / 2] = (uint32_t)bf16_bits(fmaxf(a0, 0.0f)) | ((uint32_t)bf16_bits(fmaxf(a1, 0.0f)) << 16);
+    }
+    uint4* dst = (uint4*)(y + ((size_t)Y * Wq + X) * CNN_C);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+}
+
+// 16-byte load of 8 BF16 channels, zero outside the grid
+__device__ __forceinline__ bf16x8 load_ch8(const uint16_t* t, int Hq, int Wq, int Y, int X, int c0) {
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (Y >= 0 && Y < Hq && X >= 0 && X < Wq)
+        v = *(const uint4*)(t + ((size_t)Y * Wq + X) * CNN_C + c0);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// 3x3 conv, 32 -> 32 channels, on v_mfma_f32_32x32x16_bf16: a wave computes
+// tiles of 32 consecutive cells of one row as D[co][cell] = W[co][k] x
+// In[k][cell] with k = (tap, ci) (K = 288 = 18 steps of 16). The weight
+// operands stay in registers for the wave's CNN_TILES tiles. Output lane
+// layout: lane l, accumulator i -> cell l % 32, channel 8 (i / 4) +
+// 4 (l / 32) + i % 4, stored as 4 runs of 4 channels. View = blockIdx.z.
+constexpr int CNN_TILES = 1;
+
+template<bool RELU>
+__global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restrict__ xa,
+                                                       const uint16_t* __restrict__ xb,
+                                                       const uint16_t* __restrict__ wm,
+                                                       const float* __restrict__ bias, int Hq,
+                                                       int Wq, uint16_t* __restrict__ ya,
+                                                       uint16_t* __restrict__ yb) {
+    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5;
+    const int tiles_row = (Wq + 31) / 32;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int Y = wave / ((tiles_row + CNN_TILES - 1) / CNN_TILES);
+    const int t0 = (wave % ((tiles_row + CNN_TILES - 1) / CNN_TILES)) * CNN_TILES;
+    if (Y >= Hq)
+        return;
+    const uint16_t* x = blockIdx.z ? xb : xa;
+    uint16_t* y = blockIdx.z ? yb : ya;
+    bf16x8 wa[18];
+#pragma unroll
+    for (int s = 0; s < 18; ++s)
+        wa[s] = __builtin_bit_cast(bf16x8, *(const uint4*)(wm + ((size_t)nl * 9 + (s >> 1)) * CNN_C + 16 * (s & 1) + 8 * g));
+    float bv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) bv[i] = bias[8 * (i / 4) + 4 * g + (i % 4)];
+    for (int tt = t0; tt < min(t0 + CNN_TILES, tiles_row); ++tt) {
+        const int X = tt * 32 + nl;
+        f32x16 acc = {0};
+#pragma unroll
+        for (int s = 0; s < 18; ++s) {
+            const int tap = s >> 1;
+            const bf16x8 bx = load_ch8(x, Hq, Wq, Y + tap / 3 - 1, X + tap % 3 - 1, 16 * (s & 1) + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bx, acc, 0, 0, 0);
+        }
+        if (X < Wq) {
+            uint16_t* dst = y + ((size_t)Y * Wq + X) * CNN_C + 4 * g;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float o[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float v = acc[4 * q + r] + bv[4 * q + r];
+                    o[r] = RELU ? fmaxf(v, 0.0f) : v;
+                }
+                *(uint2*)(dst + 8 * q) = make_uint2((uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16),
+                                                    (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
+            }
+        }
+    }
+}
+
+// Correlation and window of every cell of a 32-cell tile (one wave): the
+// scores <fL(X), fR(X')> of the tile's cells against the right cells
+// X' = X - (D0 + k) come from v_mfma_f32_32x32x16_bf16 over right tiles of
+// 32 cells, land in LDS as s[cell][k], and lanes 0..31 then take the
+// argmax, the softmax peak probability and the refined centre exactly as
+// k_cnn_window does. K <= CNN_KMAX_MFMA.
+constexpr int CNN_KMAX_MFMA = 160;
+
+__global__ __launch_bounds__(256) void k_cnn_corr_window(const uint16_t* __restrict__ fl,
+                                                         const uint16_t* __restrict__ fr, int Hq,
+                                                         int Wq, int K, float p_conf, int hw_lo,
+                                                         int hw_hi, float* __restrict__ center,
+                                                         int16_t* __restrict__ halfwidth,
+                                                         float* __restrict__ peak_prob) {
+    __shared__ float sc[4][32][CNN_KMAX_MFMA + 1];
+    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5, wv = threadIdx.x >> 6;
+    const int tiles_row = (Wq + 31) / 32;
+    const int wave = blockIdx.x * 4 + wv;
+    const int Y = wave / tiles_row, X0 = (wave % tiles_row) * 32;
+    if (Y >= Hq)
+        return;
+    float (&S)[32][CNN_KMAX_MFMA + 1] = sc[wv];
+    for (int i = lane; i < 32 * K; i += 64) S[i / K][i % K] = -1e30f;
+    // left operands: cell X0 + nl, channels 16 h + 8 g .. +7
+    const bf16x8 a0 = load_ch8(fl, Hq, Wq, Y, X0 + nl, 8 * g);
+    const bf16x8 a1 = load_ch8(fl, Hq, Wq, Y, X0 + nl, 16 + 8 * g);
+    // right cells span [X0 - D0 - K + 1, X0 + 31 - D0]
+    const int rb = X0 - CNN_D0 - K + 1;
+    const int nt = (31 + K + 31) / 32;
+    for (int j = 0; j < nt; ++j) {
+        const int Xr = rb + 32 * j + nl;
+        const bf16x8 b0 = load_ch8(fr, Hq, Wq, Y, Xr, 8 * g);
+        const bf16x8 b1 = load_ch8(fr, Hq, Wq, Y, Xr, 16 + 8 * g);
+        f32x16 acc = {0};
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
+        if (Xr >= 0 && Xr < Wq) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int m = 8 * (i / 4) + 4 * g + (i % 4);
+                const int k = (X0 + m) - Xr - CNN_D0;
+                if (k >= 0 && k < K) S[m][k] = acc[i] * 0.17677669529663687f;  // 1 / sqrt(32)
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int X = X0 + lane;
+    if (lane >= 32 || X >= Wq)
+        return;
+    int kb = 0;
+    float mx = -1e30f;
+    for (int k = 0; k < K; ++k) {
+        const float c = S[lane][k];
+        if (c > mx) { mx = c; kb = k; }
+    }
+    float z = 0.0f;
+    for (int k = 0; k < K; ++k) {
+        const float c = S[lane][k];
+        z += c > -1e29f ? __expf(c - mx) : 0.0f;
+    }
+    const float pk = 1.0f / z;
+    float kf = (float)kb;
+    if (kb > 0 && kb < K - 1) {
+        const float cm = S[lane][kb - 1], cp = S[lane][kb + 1];
+        if (cm > -1e29f && cp > -1e29f) {
+            const float den = cm - 2.0f * mx + cp;
+            if (den < 0.0f) kf += fminf(fmaxf(0.5f * (cm - cp) / den, -0.5f), 0.5f);
+        }
+    }
+    const size_t at = (size_t)Y * Wq + X;
     center[at] = CNN_DS * (CNN_D0 + kf);
     halfwidth[at] = (int16_t)(pk >= p_conf ? hw_lo : hw_hi);
     if (peak_prob) peak_prob[at] = pk;

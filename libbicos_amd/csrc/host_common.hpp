@@ -84,6 +84,7 @@ struct StackInfo {
 };
 
 int elem_size(int type);
+bool precision_known(int precision);
 int required_bits(uint32_t n, int mode);
 int words_for_bits(int bits);
 int check_stack(const bicos_image* s, uint32_t n, StackInfo* info, FrameTable* ft);
@@ -97,6 +98,9 @@ int check_out(const bicos_image* img, int rows, int cols, int type, const char* 
 
 /* ------------------------------------------------------ stage dispatch */
 
+// Row-search implementation (BICOS_SEARCH_MFMA or BICOS_SEARCH_VALU).
+extern int g_search_impl;
+
 int transform_any(const Stack& f0, const Stack* f1, int depth, uint32_t n, int mode,
                   int W, int H, int words, uint32_t* out0, uint32_t* out1, hipStream_t s);
 int search_flags(const uint32_t* d0, const uint32_t* d1, int H, int W, int words, int flags,
@@ -104,20 +108,54 @@ int search_flags(const uint32_t* d0, const uint32_t* d1, int H, int W, int words
 int run_agree(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
               uint32_t n, int W, int H, float thr, int has_mv, float mv_n, int precision,
               void* corr, int64_t cstep, hipStream_t s);
+// Scratch of the MOMENTS subpixel stage: per-pixel status (ST_PENDING,
+// ST_HOPELESS) and the per-row list of pending pixels.
+struct MomentsScratch {
+    uint8_t* status;    // rows * cols
+    int16_t* list;      // rows * cols
+    int* row_count;     // rows
+    static size_t bytes(size_t npx, int rows) {
+        return align_up(npx, 256) + align_up(npx * 2, 256) + align_up((size_t)rows * 4, 256);
+    }
+    static MomentsScratch carve(void* p, size_t npx) {
+        uint8_t* b = (uint8_t*)p;
+        MomentsScratch m;
+        m.status = b;
+        m.list = (int16_t*)(b + align_up(npx, 256));
+        m.row_count = (int*)(b + align_up(npx, 256) + align_up(npx * 2, 256));
+        return m;
+    }
+};
+
+// Per-row lists (row_count[r] columns at list + r * W) of the pixels whose
+// MOMENTS status has a bit of `mask`, or (mask 0) of the fallback set:
+// res NaN and not hopeless.
+int run_status_list(const uint8_t* status, const float* res, int64_t rstep, int mask, int W, int H,
+                    int16_t* list, int* row_count, hipStream_t s);
+
+// MOMENTS stage of listed pixels (row r: row_count[r] columns at
+// list + r * W), writing their status, then the exact stage on those left
+// pending
+int run_moments_listed(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                       int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                       float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
+                       const int16_t* list, const int* row_count, hipStream_t s);
+
+// ms: required for PRECISION_MOMENTS without a corrmap (with a corrmap the
+// exact FP32 stage runs)
 int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                  int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
                  float mv_n, int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
-                 hipStream_t s);
-// NCC stage of the listed pixels only (entries row << 16 | col at
-// list[0 .. *count), `blocks` x 256 lanes, grid-stride): exact FP32
-// arithmetic, FP64 for PRECISION_DOUBLE.
+                 hipStream_t s, const MomentsScratch* ms = nullptr);
+// NCC stage of listed pixels only (row r: row_count[r] columns at
+// list + r * W): exact FP32 arithmetic, FP64 for PRECISION_DOUBLE.
 int run_agree_list(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
                    uint32_t n, int W, float thr, int has_mv, float mv_n, int precision, void* corr,
-                   int64_t cstep, const uint32_t* list, const unsigned int* count, int blocks,
+                   int64_t cstep, const int16_t* list, const int* row_count, int H,
                    hipStream_t s);
 int run_subpixel_list(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                       int depth, uint32_t n, int W, float thr, float step, int has_mv, float mv_n,
                       int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
-                      const uint32_t* list, const unsigned int* count, int blocks, hipStream_t s);
+                      const int16_t* list, const int* row_count, int H, hipStream_t s);
 
 }  // namespace bicos_hip

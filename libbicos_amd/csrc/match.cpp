@@ -107,6 +107,7 @@ void match(const std::vector<Image>& stack0, const std::vector<Image>& stack1, I
     c.mode = cfg.mode == TransformMode::FULL ? BICOS_MODE_FULL : BICOS_MODE_LIMITED;
     c.precision = cfg.precision == Precision::DOUBLE      ? BICOS_PRECISION_DOUBLE
                 : cfg.precision == Precision::HALF_MFMA ? BICOS_PRECISION_HALF_MFMA
+                : cfg.precision == Precision::MOMENTS   ? BICOS_PRECISION_MOMENTS
                                                         : BICOS_PRECISION_SINGLE;
     if (std::holds_alternative<Variant::Consistency>(cfg.variant)) {
         const auto& v = std::get<Variant::Consistency>(cfg.variant);

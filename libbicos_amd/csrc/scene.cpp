@@ -12,10 +12,12 @@
  *              Gaussian blur (sigma 1.5, radius 5, separable), min-max scaled
  *              to [lo, hi];
  *   views      L_t(x,y) = P_t(x,y); R_t(x',y) = P_t(x,y) where x - D(x,y) = x'
- *              (solved per right pixel by fixed-point iteration on the row,
- *              linear interpolation of D and P), so that left pixel x sees
- *              the same surface point as right pixel x - D(x,y) (BICOS
- *              convention d = col0 - col1);
+ *              for the closest surface point landing on x' (forward warp of
+ *              the row, linear between samples, with a depth test), so that
+ *              left pixel x sees the same surface point as right pixel
+ *              x - D(x,y) (BICOS convention d = col0 - col1); right pixels
+ *              no surface lands on (disocclusions) see canvas columns no
+ *              left pixel sees;
  *   noise      i.i.d. N(0, sigma) per pixel, frame and view, rounded half to
  *              even and clamped;
  *   dark       a Bernoulli(p_dark) set of left pixels is held at a constant
@@ -81,25 +83,15 @@ void generate(int W, int H, int n, double dmin, double dmax, uint64_t seed, doub
             }
     }
 
-    /* left coordinate seen by each right pixel: x = x' + D(x), fixed point */
+    /* left coordinate seen by each right pixel (forward warp, depth test) */
+    const int CW0 = W + (int)std::ceil(dmax) + 8; /* canvas columns left pixels can see */
     std::vector<float> src((size_t)W * H);
     parallel_for(H, threads, [&](int y) {
-        const float* Dr = D.data() + (size_t)y * W;
-        auto Dat = [&](double x) {
-            x = std::min(std::max(x, 0.0), (double)(W - 1));
-            int x0 = (int)std::floor(x);
-            int x1 = std::min(x0 + 1, W - 1);
-            double w = x - x0;
-            return (1.0 - w) * Dr[x0] + w * Dr[x1];
-        };
-        for (int xr = 0; xr < W; ++xr) {
-            double x = xr + (double)Dr[xr];
-            for (int it = 0; it < 12; ++it) x = xr + Dat(x);
-            src[(size_t)y * W + xr] = (float)x;
-        }
+        std::vector<double> zbuf(W);
+        warp_row(D.data() + (size_t)y * W, W, (double)CW0, src.data() + (size_t)y * W, zbuf.data());
     });
 
-    const int CW = W + (int)std::ceil(dmax) + 8; /* canvas width */
+    const int CW = CW0 + W; /* canvas width: + a region only disoccluded right pixels see */
     const int R = BLUR_R;
     double g[2 * R + 1];
     blur_taps(g);

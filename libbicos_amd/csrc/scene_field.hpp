@@ -85,6 +85,35 @@ SCENE_HD inline double gauss_noise(uint64_t h) {
     return sqrt(-2.0 * log(u1)) * cos(2.0 * SCENE_PI * u2);
 }
 
+/* Right view of one row by forward warping the left row with a depth test:
+ * the left row, linear between samples, lands at right coordinates
+ * x - D(x); every right pixel x' gets the left coordinate of the closest
+ * (largest-disparity) surface point that lands on it, and a right pixel no
+ * forward-facing surface lands on (a disocclusion) gets the coordinate
+ * right_only + x' of a canvas region no left pixel sees. zbuf: W scratch
+ * values. Pure +, -, *, / and ceil, so host and device agree exactly. */
+SCENE_HD inline void warp_row(const float* Dr, int W, double right_only, float* src, double* zbuf) {
+    for (int xr = 0; xr < W; ++xr) {
+        zbuf[xr] = -1e300;
+        src[xr] = (float)(right_only + xr);
+    }
+    for (int x = 0; x + 1 < W; ++x) {
+        const double a = x - (double)Dr[x], b = (x + 1) - (double)Dr[x + 1];
+        if (!(b > a)) continue;  // folded away from the right camera
+        int lo = (int)ceil(a), hi = (int)ceil(b) - 1;  // right pixels in [a, b)
+        if (lo < 0) lo = 0;
+        if (hi > W - 1) hi = W - 1;
+        for (int xr = lo; xr <= hi; ++xr) {
+            const double t = (xr - a) / (b - a);
+            const double d = (1.0 - t) * (double)Dr[x] + t * (double)Dr[x + 1];
+            if (d > zbuf[xr]) {
+                zbuf[xr] = d;
+                src[xr] = (float)(x + t);
+            }
+        }
+    }
+}
+
 /* normalised Gaussian blur taps */
 inline void blur_taps(double g[2 * BLUR_R + 1]) {
     double gs = 0.0;

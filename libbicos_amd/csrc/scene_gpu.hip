@@ -104,22 +104,13 @@ __global__ __launch_bounds__(256) void k_scene_dark(uint64_t seed, const float* 
     }
 }
 
-// left coordinate seen by each right pixel: x = x' + D(x), fixed point
-__global__ __launch_bounds__(256) void k_scene_src(const float* __restrict__ D, int W, int H,
-                                                   float* __restrict__ src) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (size_t)W * H) return;
-    const int y = (int)(i / W), xr = (int)(i % W);
-    const float* Dr = D + (size_t)y * W;
-    double x = xr + (double)Dr[xr];
-    for (int it = 0; it < 12; ++it) {
-        double c = fmin(fmax(x, 0.0), (double)(W - 1));
-        const int x0 = (int)floor(c);
-        const int x1 = min(x0 + 1, W - 1);
-        const double w = c - x0;
-        x = xr + ((1.0 - w) * Dr[x0] + w * Dr[x1]);
-    }
-    src[i] = (float)x;
+// left coordinate seen by each right pixel: one lane per row (warp_row)
+__global__ __launch_bounds__(64) void k_scene_src(const float* __restrict__ D, int W, int H,
+                                                  double right_only, float* __restrict__ src,
+                                                  double* __restrict__ zbuf) {
+    const int y = blockIdx.x * 64 + threadIdx.x;
+    if (y >= H) return;
+    warp_row(D + (size_t)y * W, W, right_only, src + (size_t)y * W, zbuf + (size_t)y * W);
 }
 
 __global__ __launch_bounds__(256) void k_scene_noise(uint64_t seed, int H, int CW, float* __restrict__ noise) {
@@ -209,11 +200,13 @@ extern "C" int bicos_hip_scene_generate(int32_t W, int32_t H, int32_t n, int32_t
         return fail(BICOS_ERR_INVALID_ARGUMENT, "bad scene parameters");
     hipStream_t s = (hipStream_t)stream;
     const Field f = make_field(seed, W, H);
-    const int CW = W + (int)std::ceil(dmax) + 8;  // canvas width
+    const int CW0 = W + (int)std::ceil(dmax) + 8;  // canvas columns left pixels can see
+    const int CW = CW0 + W;                          // + a region only disoccluded right pixels see
     Taps taps;
     blur_taps(taps.g);
 
     const size_t npx = (size_t)W * H, ncv = (size_t)H * CW;
+    // raw: the field values, later the warp's depth buffer
     const size_t b_raw = align_up(npx * 8, 256), b_D = align_up(npx * 4, 256),
                  b_dark = align_up(npx, 256), b_src = align_up(npx * 4, 256),
                  b_cv = align_up((size_t)n * ncv * 4, 256),
@@ -244,7 +237,7 @@ extern "C" int bicos_hip_scene_generate(int32_t W, int32_t H, int32_t n, int32_t
         LAUNCH("scene", k_scene_disp, gpx, blk, 0, s, (const double*)raw, (const double*)mm, npx, dmin, dmax, D);
         HIP_TRY(hipMemsetAsync(darkR, 0, npx, s));
         LAUNCH("scene", k_scene_dark, gpx, blk, 0, s, seed, (const float*)D, W, H, p_dark, darkL, darkR);
-        LAUNCH("scene", k_scene_src, gpx, blk, 0, s, (const float*)D, W, H, src);
+        LAUNCH("scene", k_scene_src, dim3((H + 63) / 64), dim3(64), 0, s, (const float*)D, W, H, (double)CW0, src, raw);
         LAUNCH("scene", k_scene_noise, gcv, blk, 0, s, seed, H, CW, cv0);
         LAUNCH("scene", k_scene_hblur, gcv, blk, 0, s, (const float*)cv0, taps, H, CW, cv1);
         LAUNCH("scene", k_scene_vblur, gcv, blk, 0, s, (const float*)cv1, taps, H, CW, cv0);

@@ -59,31 +59,49 @@ __device__ __forceinline__ v4i expand_pm1(uint32_t b16) {
 // that the state offsets (at most 32 * 1024 tiles + 4) cannot wrap it around.
 constexpr uint32_t NO_KEY = 0x7FFFFFFFu;
 
-template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH>
+// LIST = true: the queries of row r are the listed columns
+// qcol[r * W + i], i < row_count[r] (negative entries are skipped); the row
+// is blockIdx.x so that rows, most of which need one block, spread over the
+// XCDs. Results go to out[r * ostep + col] (SEARCH_DISP) or to
+// res[r * W + i] (SEARCH_REV). Waves without listed queries skip the
+// arithmetic but take part in the staging.
+template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH, bool LIST = false>
 __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
     const uint32_t* __restrict__ dq, const uint32_t* __restrict__ dc, int W,
-    const int16_t* __restrict__ rev, int max_lr_diff, int16_t* __restrict__ out, int64_t ostep) {
+    const int16_t* __restrict__ rev, int max_lr_diff, int16_t* __restrict__ out, int64_t ostep,
+    const int16_t* __restrict__ qcol, const int* __restrict__ row_count, int16_t* __restrict__ res) {
     constexpr int KS = WORDS;              // K steps of 32 bits
     constexpr int STRIDE = 2 * KS + 1;     // uint4 per staged candidate (16 B pad)
     extern __shared__ uint4 lds[];
 
-    const int row = blockIdx.y;
+    const int row = LIST ? blockIdx.x : blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nl = lane & 31, g = lane >> 5;
+    const int group = LIST ? blockIdx.y : blockIdx.x;
+    int count = 0;
+    if constexpr (LIST) {
+        count = row_count[row];
+        if (group * WAVES * NB * 32 >= count)
+            return;  // uniform over the block
+    }
 
     // ---- query operands: bits 32 s + 16 g .. +15 of each query pixel
     v4i qop[NB][KS];
     uint32_t s1[NB], s2[NB];
-    const int qbase = (blockIdx.x * WAVES + wave) * (NB * 32);
+    int qc[NB];
+    const int qbase = (group * WAVES + wave) * (NB * 32);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int col = qbase + b * 32 + nl;
-        const uint32_t* src = dq + ((size_t)row * W + (col < W ? col : 0)) * WORDS;
+        int col = qbase + b * 32 + nl;
+        if constexpr (LIST) col = col < count ? qcol[(size_t)row * W + col] : -1;
+        qc[b] = col;
+        const uint32_t* src = dq + ((size_t)row * W + (col >= 0 && col < W ? col : 0)) * WORDS;
 #pragma unroll
         for (int s = 0; s < KS; ++s) qop[b][s] = expand_pm1(src[s] >> (16 * g));
         s1[b] = NO_KEY;
         s2[b] = NO_KEY;
     }
+    const bool busy = !LIST || qbase < count;
 
     const uint32_t* crow = dc + (size_t)row * W * WORDS;
     const int ntiles = (W + 31) / 32;
@@ -102,7 +120,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
         // full tiles: three VALU ops per output; the NB product chains are
         // independent, so matrix-core work of one overlaps the VALU work of
         // the previous
-        const int nfull = cn / 32;
+        const int nfull = busy ? cn / 32 : 0;
         for (int mt = 0; mt < nfull; ++mt) {
             v4i aop[KS];
 #pragma unroll
@@ -133,7 +151,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
             }
         }
         // ragged last tile of the row: candidates at or beyond W are masked
-        if (nfull * 32 < cn) {
+        if (busy && nfull * 32 < cn) {
             const int mt = nfull;
             v4i aop[KS];
 #pragma unroll
@@ -170,7 +188,19 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
         const uint32_t o2 = (uint32_t)__shfl_xor((int)b2, 32);
         const uint32_t m1 = min(b1, o1);
         const uint32_t m2 = min(max(b1, o1), min(b2, o2));
-        const int col = qbase + b * 32 + nl;
+        const int col = qc[b];
+        if constexpr (LIST) {
+            const int i = qbase + b * 32 + nl;
+            if (g != 0 || i >= count)
+                continue;
+            const bool valid = col >= 0 && !(NODUPES && (m1 >> 16) == (m2 >> 16));
+            const int16_t best = valid ? (int16_t)(m1 & 0xFFFFu) : INVALID16;
+            if constexpr (MODE == SEARCH_REV)
+                res[(size_t)row * W + i] = best;
+            else if (col >= 0)
+                out[(size_t)row * ostep + col] = valid ? (int16_t)(col - best) : INVALID16;
+            continue;
+        }
         if (g != 0 || col >= W)
             continue;
         const int best = (int)(m1 & 0xFFFFu);

@@ -33,6 +33,8 @@ int check_options(const bicos_window_options* o, int depth, float* in_scale, int
         return fail(BICOS_ERR_INVALID_ARGUMENT, "window bins must be in [9, 1024]");
     if (o->hw_lo < 0 || o->hw_hi < 0 || o->hw_lo > 32767 || o->hw_hi > 32767)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "window half-widths must be in [0, 32767]");
+    if (o->fallback != 0 && o->fallback != 1)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "fallback must be 0 or 1");
     if (!(o->in_scale >= 0.0f) || !std::isfinite(o->in_scale))
         return fail(BICOS_ERR_INVALID_ARGUMENT, "bad input scale");
     *in_scale = o->in_scale > 0.0f ? o->in_scale : 8.0f / (depth == BICOS_8U ? 255.0f : 65535.0f);
@@ -136,12 +138,67 @@ size_t partial_floats(const Grid& g) {
     return (size_t)((g.Hq + CNN_BANDS_ROWS - 1) / CNN_BANDS_ROWS) * CNN_C * (CNN_C * 9 + 1);
 }
 
-// Windows of all cells into center/halfwidth (peak nullable); the buffers
-// after `p` hold wbf, both views' activations and the correlation volume.
+// Inference implementation of the network: matrix cores (default) or the
+// training kernels on the vector ALU (for tests and measurement).
+int g_cnn_impl = BICOS_CNN_MFMA;
+
+size_t valu_scratch_bytes(const Grid& g) {
+    return 4 * (align_up(CNN_PARAMS, 64) + 2 * acts_floats(g.cells) + (size_t)g.K * g.cells);
+}
+
+size_t mfma_scratch_bytes(const Grid& g) {
+    return align_up(CNN_W1 * 4, 256) + align_up(CNN_MFMA_W * 2, 256) +
+           2 * align_up(CNN_IN * g.cells * 4, 256) + 4 * align_up(CNN_C * g.cells * 2, 256);
+}
+
+size_t windows_scratch_bytes(const Grid& g) {
+    return std::max(valu_scratch_bytes(g), mfma_scratch_bytes(g)) + 256;
+}
+
+// Windows of all cells into center/halfwidth (peak nullable); scratch holds
+// windows_scratch_bytes(g).
 int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, float in_scale,
-                const float* params, const bicos_window_options* o, float* scratch, float* center,
+                const float* params, const bicos_window_options* o, void* scratch, float* center,
                 int16_t* halfwidth, float* peak, hipStream_t s) {
-    float* wbf = scratch;
+    const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
+    if (g_cnn_impl == BICOS_CNN_MFMA) {
+        if (g.K > CNN_KMAX_MFMA)
+            return fail(BICOS_ERR_INVALID_ARGUMENT, "window bins above 160 need the vector-ALU network");
+        uint8_t* p = (uint8_t*)scratch;
+        float* w0 = (float*)p; p += align_up(CNN_W1 * 4, 256);
+        uint16_t* wm = (uint16_t*)p; p += align_up(CNN_MFMA_W * 2, 256);
+        float* x0L = (float*)p; p += align_up(CNN_IN * g.cells * 4, 256);
+        float* x0R = (float*)p; p += align_up(CNN_IN * g.cells * 4, 256);
+        uint16_t* aL = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
+        uint16_t* bL = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
+        uint16_t* aR = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
+        uint16_t* bR = (uint16_t*)p;
+        LAUNCH("cnn_weights", k_cnn_pack_mfma, dim3((CNN_MFMA_W + 255) / 256), block, 0, s, params, w0, wm);
+        if (depth == BICOS_8U) {
+            LAUNCH("cnn_input", k_cnn_input<uint8_t>, grid, block, 0, s, v0, g.W, g.Hq, g.Wq, in_scale, x0L);
+            LAUNCH("cnn_input", k_cnn_input<uint8_t>, grid, block, 0, s, v1, g.W, g.Hq, g.Wq, in_scale, x0R);
+        } else {
+            LAUNCH("cnn_input", k_cnn_input<uint16_t>, grid, block, 0, s, v0, g.W, g.Hq, g.Wq, in_scale, x0L);
+            LAUNCH("cnn_input", k_cnn_input<uint16_t>, grid, block, 0, s, v1, g.W, g.Hq, g.Wq, in_scale, x0R);
+        }
+        LAUNCH("cnn_conv", k_cnn_conv0_cm, dim3((g.Wq + 255) / 256, g.Hq, 2), block, 0, s,
+               (const float*)x0L, (const float*)x0R, (const float*)w0, params + cnn_b_off(0), g.Hq, g.Wq, aL, aR);
+        const int tiles_row = (g.Wq + 31) / 32;
+        const int waves = g.Hq * ((tiles_row + CNN_TILES - 1) / CNN_TILES);
+        const dim3 cgrid((waves + 3) / 4, 1, 2);
+        const size_t wl = (size_t)CNN_C * 9 * CNN_C;
+        LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, block, 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
+               (const uint16_t*)wm, params + cnn_b_off(1), g.Hq, g.Wq, bL, bR);
+        LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, block, 0, s, (const uint16_t*)bL, (const uint16_t*)bR,
+               (const uint16_t*)(wm + wl), params + cnn_b_off(2), g.Hq, g.Wq, aL, aR);
+        LAUNCH("cnn_conv", k_cnn_conv_mfma<false>, cgrid, block, 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
+               (const uint16_t*)(wm + 2 * wl), params + cnn_b_off(3), g.Hq, g.Wq, bL, bR);
+        const int cw = g.Hq * tiles_row;
+        LAUNCH("cnn_corr_window", k_cnn_corr_window, dim3((cw + 3) / 4), block, 0, s, (const uint16_t*)bL,
+               (const uint16_t*)bR, g.Hq, g.Wq, g.K, o->p_conf, o->hw_lo, o->hw_hi, center, halfwidth, peak);
+        return BICOS_OK;
+    }
+    float* wbf = (float*)scratch;
     float* actp = wbf + align_up(CNN_PARAMS, 64);
     const ViewActs aL = carve_acts(actp, g.cells);
     const ViewActs aR = carve_acts(actp + acts_floats(g.cells), g.cells);
@@ -150,15 +207,10 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
     if ((rc = run_bf16_weights(params, wbf, s))) return rc;
     if ((rc = cnn_forward(v0, depth, g, in_scale, wbf, params, aL, s))) return rc;
     if ((rc = cnn_forward(v1, depth, g, in_scale, wbf, params, aR, s))) return rc;
-    const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
     LAUNCH("cnn_corr", k_cnn_corr, grid, block, 0, s, (const float*)aL.f, (const float*)aR.f, g.Hq, g.Wq, g.K, C);
     LAUNCH("cnn_window", k_cnn_window, grid, block, 0, s, (const float*)C, g.Hq, g.Wq, g.K, o->p_conf,
            o->hw_lo, o->hw_hi, center, halfwidth, peak);
     return BICOS_OK;
-}
-
-size_t windows_scratch_floats(const Grid& g) {
-    return align_up(CNN_PARAMS, 64) + 2 * acts_floats(g.cells) + (size_t)g.K * g.cells;
 }
 
 /* ------------------------------------------------------- windowed search */
@@ -196,7 +248,7 @@ int run_search_list_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool
                       int64_t dstep, hipStream_t s) {
     const int chunk = (int)std::min<size_t>((size_t)W, 64 * 1024 / (WORDS * 4));
     const size_t lds = (size_t)chunk * WORDS * 4;
-    const dim3 grid((W + SEARCH_THREADS - 1) / SEARCH_THREADS, H), block(SEARCH_THREADS);
+    const dim3 grid(H, (W + SEARCH_THREADS - 1) / SEARCH_THREADS), block(SEARCH_THREADS);
 #define SL(ND, M) LAUNCH("search_list", (k_search_list<WORDS, ND, M>), grid, block, lds, s, dq, dc, W, chunk, qcol, row_count, res, disp, dstep)
     if (nodupes) {
         if (mode == SEARCH_REV) SL(true, SEARCH_REV); else SL(true, SEARCH_DISP);
@@ -207,9 +259,36 @@ int run_search_list_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool
     return BICOS_OK;
 }
 
+template<int WORDS, int NB, int WAVES, int CH>
+int run_search_list_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes,
+                           int mode, const int16_t* qcol, const int* row_count, int16_t* res,
+                           int16_t* disp, int64_t dstep, hipStream_t s) {
+    constexpr size_t lds = (size_t)CH * (2 * WORDS + 1) * 16;
+    const dim3 grid(H, (W + WAVES * NB * 32 - 1) / (WAVES * NB * 32)), block(WAVES * 64);
+#define SM(ND, M) LAUNCH("search_list_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH, true>), grid, block, lds, s, dq, dc, W, (const int16_t*)nullptr, 0, disp, dstep, qcol, row_count, res)
+    if (nodupes) {
+        if (mode == SEARCH_REV) SM(true, SEARCH_REV); else SM(true, SEARCH_DISP);
+    } else {
+        if (mode == SEARCH_REV) SM(false, SEARCH_REV); else SM(false, SEARCH_DISP);
+    }
+#undef SM
+    return BICOS_OK;
+}
+
+// Full-row search of the listed pixels (same kernel shapes as the full
+// search of bicos_hip_match).
 int run_search_list(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
                     int mode, const int16_t* qcol, const int* row_count, int16_t* res,
                     int16_t* disp, int64_t dstep, hipStream_t s) {
+    if (g_search_impl == BICOS_SEARCH_MFMA) {
+        switch (words) {
+            case 1: return run_search_list_mfma_w<1, 2, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 2: return run_search_list_mfma_w<2, 2, 8, 800>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 4: return run_search_list_mfma_w<4, 2, 8, 448>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 8: return run_search_list_mfma_w<8, 2, 8, 224>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+        }
+    }
     switch (words) {
         case 1: return run_search_list_w<1>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
         case 2: return run_search_list_w<2>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
@@ -224,13 +303,19 @@ __global__ void k_zero_u32(unsigned int* p, int count) {
     if (i < count) p[i] = 0u;
 }
 
+// status non-null (MOMENTS): the set from the NCC stage's status bits
 int run_fallback_list(const float* disp_f, const int16_t* disp_i, int64_t dstep, const Stack& v0,
-                      int depth, uint32_t n, const Grid& g, int has_mv, float mv_n, int precision,
-                      int16_t* list, int* row_count, uint32_t* glist, unsigned int* gcount,
+                      int depth, uint32_t n, const Grid& g, int has_mv, float mv_n, float thr,
+                      int precision, const uint8_t* status, int16_t* list, int* row_count,
                       hipStream_t s) {
-    LAUNCH("fallback_list", k_zero_u32, dim3(1), dim3(256), 0, s, gcount, 1);
-    const dim3 grid(g.H), block(256);
-#define FL(TI, TP) LAUNCH("fallback_list", (k_fallback_list<TI, TP>), grid, block, 0, s, disp_f, disp_i, dstep, v0, (int)n, g.W, has_mv, mv_n, list, row_count, glist, gcount)
+    LAUNCH("fallback_list", k_zero_u32, dim3((g.H + 255) / 256), dim3(256), 0, s, (unsigned int*)row_count, g.H);
+    const dim3 grid(g.H, (g.W + 255) / 256), block(256);
+    if (status) {
+        LAUNCH("fallback_list", k_fallback_list_status, grid, block, 0, s, status, disp_f, dstep, g.W,
+               list, row_count);
+        return BICOS_OK;
+    }
+#define FL(TI, TP) LAUNCH("fallback_list", (k_fallback_list<TI, TP>), grid, block, 0, s, disp_f, disp_i, dstep, v0, (int)n, g.W, has_mv, mv_n, thr, list, row_count)
     if (precision == BICOS_PRECISION_DOUBLE) {
         if (depth == BICOS_8U) FL(uint8_t, double); else FL(uint16_t, double);
     } else {
@@ -248,6 +333,13 @@ int run_fallback_list(const float* disp_f, const int16_t* disp_i, int64_t dstep,
 extern "C" {
 
 int bicos_hip_window_param_count(void) { return CNN_PARAMS; }
+
+int bicos_hip_set_cnn_impl(int32_t impl) {
+    if (impl != BICOS_CNN_MFMA && impl != BICOS_CNN_VALU)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown network implementation");
+    g_cnn_impl = impl;
+    return BICOS_OK;
+}
 
 void bicos_hip_window_options_default(bicos_window_options* o) {
     if (!o) return;
@@ -380,15 +472,15 @@ int bicos_hip_window_predict(const bicos_image* stack0, const bicos_image* stack
     if ((rc = current_device(&dev))) return rc;
     std::lock_guard<std::mutex> lk(g_ws_mu);
     void* ws;
-    if ((rc = ws_acquire(dev, windows_scratch_floats(g) * 4 + pk0 + pk1 + 1024, s, &ws, WS_WINDOW))) return rc;
+    if ((rc = ws_acquire(dev, windows_scratch_bytes(g) + pk0 + pk1 + 1024, s, &ws, WS_WINDOW))) return rc;
     auto body = [&]() -> int {
         int r;
         uint8_t* pb = (uint8_t*)ws;
         Stack v0, v1;
         if ((r = make_stack(f0, n, i0, pb, &v0, s))) return r;
         if ((r = make_stack(f1, n, i1, pb + pk0, &v1, s))) return r;
-        return cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, (float*)(pb + pk0 + pk1),
-                           center, halfwidth, peak, s);
+        return cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, pb + pk0 + pk1, center,
+                           halfwidth, peak, s);
     };
     rc = body();
     const int rc2 = ws_release(dev, s, WS_WINDOW);
@@ -409,8 +501,7 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown transform mode");
     if (cfg->variant != BICOS_VARIANT_NO_DUPLICATES && cfg->variant != BICOS_VARIANT_CONSISTENCY)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented");
-    if (cfg->precision != BICOS_PRECISION_SINGLE && cfg->precision != BICOS_PRECISION_DOUBLE &&
-        cfg->precision != BICOS_PRECISION_HALF_MFMA)
+    if (!precision_known(cfg->precision))
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unimplemented precision");
     float in_scale;
     Grid g;
@@ -439,18 +530,19 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
     const size_t desc_bytes = align_up(npx * words * 4, 256);
     const size_t raw_bytes = subpix ? align_up(npx * 2, 256) : 0;
     const size_t cell_bytes = align_up(g.cells * 4, 256) + align_up(g.cells * 2, 256);
-    const size_t fb_bytes = fallback ? 2 * align_up(npx * 2, 256) + align_up((size_t)H * 4, 256) +
-                                           align_up(npx * 4, 256) + 256 +
+    const bool moments = subpix && cfg->precision == BICOS_PRECISION_MOMENTS;
+    const size_t fb_bytes = fallback ? align_up(npx * 2, 256) + align_up((size_t)H * 4, 256) + 256 +
                                            (consistency ? 2 * align_up(npx * 2, 256) : 0)
                                      : 0;
-    const size_t cnn_bytes = align_up(windows_scratch_floats(g) * 4, 256);
+    const size_t mom_bytes = moments ? MomentsScratch::bytes(npx, H) : 0;
+    const size_t cnn_bytes = align_up(windows_scratch_bytes(g), 256);
     const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
 
     int dev;
     if ((rc = current_device(&dev))) return rc;
     std::lock_guard<std::mutex> lk(g_ws_mu);
     void* ws;
-    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + cell_bytes + fb_bytes + cnn_bytes + pk0 + pk1, s, &ws)))
+    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + cell_bytes + fb_bytes + mom_bytes + cnn_bytes + pk0 + pk1, s, &ws)))
         return rc;
     uint8_t* p = (uint8_t*)ws;
     uint32_t* desc0 = (uint32_t*)p; p += desc_bytes;
@@ -460,20 +552,19 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
     int16_t* hw = (int16_t*)p; p += align_up(g.cells * 2, 256);
     int16_t *list = nullptr, *fwd = nullptr, *rev = nullptr;
     int* row_count = nullptr;
-    uint32_t* glist = nullptr;
-    unsigned int* gcount = nullptr;
+    unsigned long long* fb_total = nullptr;
     if (fallback) {
         list = (int16_t*)p; p += align_up(npx * 2, 256);
-        p += align_up(npx * 2, 256);  // spare
         row_count = (int*)p; p += align_up((size_t)H * 4, 256);
-        glist = (uint32_t*)p; p += align_up(npx * 4, 256);
-        gcount = (unsigned int*)p; p += 256;
+        fb_total = (unsigned long long*)p; p += 256;
         if (consistency) {
             fwd = (int16_t*)p; p += align_up(npx * 2, 256);
             rev = (int16_t*)p; p += align_up(npx * 2, 256);
         }
     }
-    float* cnn_scratch = (float*)p; p += cnn_bytes;
+    const MomentsScratch ms = MomentsScratch::carve(p, npx);
+    p += mom_bytes;
+    void* cnn_scratch = p; p += cnn_bytes;
     uint8_t* pack0 = p;
     uint8_t* pack1 = p + pk0;
 
@@ -501,7 +592,7 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         if (subpix)
             r = run_subpixel(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                              cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision, outf,
-                             ostep, cm, cstep, s);
+                             ostep, cm, cstep, s, moments ? &ms : nullptr);
         else
             r = run_agree(d16, d16_step, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                           cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, s);
@@ -509,8 +600,9 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
 
         // fallback: full-row search and NCC stage of the rejected pixels
         if ((r = run_fallback_list(outf, subpix ? nullptr : d16, subpix ? ostep : d16_step, v0,
-                                   i0.depth, n, g, cfg->has_min_variance, mv_n, cfg->precision,
-                                   list, row_count, glist, gcount, s)))
+                                   i0.depth, n, g, cfg->has_min_variance, mv_n,
+                                   cfg->nxcorr_threshold, cfg->precision,
+                                   moments && !cm ? ms.status : nullptr, list, row_count, s)))
             return r;
         if (consistency) {
             if ((r = run_search_list(desc0, desc1, H, W, words, nodupes, SEARCH_REV, list, row_count,
@@ -519,7 +611,7 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
             if ((r = run_search_list(desc1, desc0, H, W, words, nodupes, SEARCH_REV, fwd, row_count,
                                      rev, nullptr, 0, s)))
                 return r;
-            LAUNCH("lr_combine", k_lr_combine, dim3((W + 255) / 256, H), dim3(256), 0, s,
+            LAUNCH("lr_combine", k_lr_combine, dim3(H, (W + 255) / 256), dim3(256), 0, s,
                    (const int16_t*)list, (const int*)row_count, W, (const int16_t*)fwd,
                    (const int16_t*)rev, max_lr, d16, d16_step);
         } else {
@@ -527,26 +619,31 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
                                      nullptr, d16, d16_step, s)))
                 return r;
         }
-        const int blocks = (int)std::min<size_t>((npx + 255) / 256, 2048);
+        if (stats)
+            LAUNCH("fallback_list", k_sum_counts, dim3(1), dim3(256), 0, s, (const int*)row_count, H, fb_total);
+        if (moments && !cm)
+            return run_moments_listed(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                                      cfg->subpixel_step, cfg->has_min_variance, mv_n, outf, ostep,
+                                      &ms, list, row_count, s);
         if (subpix)
             return run_subpixel_list(raw, W, v0, v1, i0.depth, n, W, cfg->nxcorr_threshold,
                                      cfg->subpixel_step, cfg->has_min_variance, mv_n,
-                                     cfg->precision, outf, ostep, cm, cstep, glist, gcount, blocks, s);
+                                     cfg->precision, outf, ostep, cm, cstep, list, row_count, H, s);
         return run_agree_list(d16, d16_step, v0, v1, i0.depth, n, W, cfg->nxcorr_threshold,
-                              cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, glist, gcount,
-                              blocks, s);
+                              cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, list,
+                              row_count, H, s);
     };
     rc = body();
-    unsigned int fb_count = 0;
+    unsigned long long fb_count = 0;
     if (!rc && stats && fallback) {
-        const hipError_t e1 = hipMemcpyAsync(&fb_count, gcount, 4, hipMemcpyDeviceToHost, s);
+        const hipError_t e1 = hipMemcpyAsync(&fb_count, fb_total, 8, hipMemcpyDeviceToHost, s);
         const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(s) : e1;
         if (e2 != hipSuccess) rc = fail(BICOS_ERR_HIP, std::string("fallback count: ") + hipGetErrorString(e2));
     }
     const int rc2 = ws_release(dev, s);
     if (!rc && stats) {
         stats->pixels = (int64_t)npx;
-        stats->fallback_pixels = fb_count;
+        stats->fallback_pixels = (int64_t)fb_count;
     }
     return rc ? rc : rc2;
 }

@@ -14,6 +14,7 @@
 #include "cnn.hpp"
 #include "device_common.hpp"
 #include "search.hpp"
+#include "search_mfma.hpp"
 
 namespace bicos_hip {
 
@@ -93,61 +94,87 @@ __global__ __launch_bounds__(256) void k_window_search(const uint32_t* __restric
 
 // Fallback set: pixels whose windowed result is invalid and whose left
 // variance does not already reject every match (has_minvar and
-// var0 < minvar, computed as the NCC stage does in precision TPrec). Writes
-// the per-row lists (columns ascending, row_count[row] entries at
-// list + row * W) and the packed (row << 16 | col) entries of the whole set
-// at glist[0 .. *gcount).
+// var0 < minvar, computed as the NCC stage does in precision TPrec). Each
+// block takes 256 columns of one row (the row is blockIdx.x so that rows
+// spread over the XCDs) and appends its pixels to the row's list
+// (row_count[row] entries at list + row * W, in no particular order);
+// row_count starts at zero.
 template<typename TIn, typename TPrec>
 __global__ __launch_bounds__(256) void k_fallback_list(const float* __restrict__ disp_f,
                                                        const int16_t* __restrict__ disp_i,
                                                        int64_t dstep, const Stack f0, int n, int W,
-                                                       int has_minvar, float minvar,
+                                                       int has_minvar, float minvar, float thr,
                                                        int16_t* __restrict__ list,
-                                                       int* __restrict__ row_count,
-                                                       uint32_t* __restrict__ glist,
-                                                       unsigned int* __restrict__ gcount) {
-    __shared__ int base, gbase;
-    __shared__ int wsum[4];
-    const int row = blockIdx.x;
+                                                       int* __restrict__ row_count) {
+    __shared__ int wsum[4], base;
+    const int row = blockIdx.x, col = blockIdx.y * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) base = 0;
-    __syncthreads();
-    for (int c0 = 0; c0 < W; c0 += 256) {
-        const int col = c0 + threadIdx.x;
-        bool need = false;
-        if (col < W) {
-            const size_t i = (size_t)row * dstep + col;
-            need = disp_f ? __builtin_isnan(disp_f[i]) : disp_i[i] == INVALID16;
-            if (need && has_minvar) {
-                TPrec mean = 0;
-                for (int t = 0; t < n; ++t) mean += (TPrec)load_sample_cached<TIn>(f0, t, row, col);
-                mean /= (TPrec)n;
-                TPrec var = 0;
-                for (int t = 0; t < n; ++t) {
-                    const TPrec d = (TPrec)load_sample_cached<TIn>(f0, t, row, col) - mean;
+    bool need = false;
+    if (col < W) {
+        const size_t i = (size_t)row * dstep + col;
+        need = disp_f ? __builtin_isnan(disp_f[i]) : disp_i[i] == INVALID16;
+        if (need && has_minvar && thr > -1.0f) {
+            // var0 as the NCC stage forms it: mean, then an fma chain of the
+            // centred samples in frame order
+            float v[MAX_FRAMES];
+#pragma unroll
+            for (int t = 0; t < MAX_FRAMES; ++t)
+                v[t] = t < n ? (float)load_sample_cached<TIn>(f0, t, row, col) : 0.0f;
+            TPrec mean = 0;
+#pragma unroll
+            for (int t = 0; t < MAX_FRAMES; ++t)
+                if (t < n) mean += (TPrec)v[t];
+            mean /= (TPrec)n;
+            TPrec var = 0;
+#pragma unroll
+            for (int t = 0; t < MAX_FRAMES; ++t)
+                if (t < n) {
+                    const TPrec d = (TPrec)v[t] - mean;
                     var = pfma(d, d, var);
                 }
-                need = !(var < (TPrec)minvar);
-            }
+            need = !(var < (TPrec)minvar);
         }
-        const uint64_t m = __ballot(need);
-        const int before = __popcll(m & ((1ull << lane) - 1));
-        if (lane == 0) wsum[wave] = __popcll(m);
-        __syncthreads();
-        int off = base;
-        for (int w = 0; w < wave; ++w) off += wsum[w];
-        if (need) list[(size_t)row * W + off + before] = (int16_t)col;
-        __syncthreads();
-        if (threadIdx.x == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        __syncthreads();
     }
+    const uint64_t m = __ballot(need);
+    if (lane == 0) wsum[wave] = __popcll(m);
+    __syncthreads();
     if (threadIdx.x == 0) {
-        row_count[row] = base;
-        gbase = (int)atomicAdd(gcount, (unsigned int)base);
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        base = total ? atomicAdd(&row_count[row], total) : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < base; i += 256)
-        glist[gbase + i] = ((uint32_t)row << 16) | (uint32_t)(uint16_t)list[(size_t)row * W + i];
+    if (need) {
+        int off = __popcll(m & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        list[(size_t)row * W + base + off] = (int16_t)col;
+    }
+}
+
+// Fallback set from the MOMENTS status: invalid result and not hopeless
+// (ST_HOPELESS = 2). As k_fallback_list without touching the stacks.
+__global__ __launch_bounds__(256) void k_fallback_list_status(
+    const uint8_t* __restrict__ status, const float* __restrict__ res, int64_t rstep, int W,
+    int16_t* __restrict__ list, int* __restrict__ row_count) {
+    __shared__ int wsum[4], base;
+    const int row = blockIdx.x, col = blockIdx.y * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool take = false;
+    if (col < W) {
+        take = __builtin_isnan(res[(size_t)row * rstep + col]) && !(status[(size_t)row * W + col] & 2);
+    }
+    const uint64_t m = __ballot(take);
+    if (lane == 0) wsum[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        base = total ? atomicAdd(&row_count[row], total) : 0;
+    }
+    __syncthreads();
+    if (take) {
+        int off = __popcll(m & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        list[(size_t)row * W + base + off] = (int16_t)col;
+    }
 }
 
 // Full-row search of the listed query columns of each row (the row's
@@ -157,7 +184,10 @@ __global__ __launch_bounds__(256) void k_fallback_list(const float* __restrict__
 //                     to res[r * W + i]
 //   MODE SEARCH_DISP: col - best to disp[r * dstep + col]
 // One block serves 256 entries of one row; waves past the row's count skip
-// the arithmetic but take part in the staging.
+// the arithmetic but take part in the staging. The row is blockIdx.x:
+// consecutive workgroups go to different XCDs, and most rows need only
+// their first block, so rows (not entry blocks) must be what the
+// round-robin spreads.
 template<int WORDS, bool NODUPES, int MODE>
 __global__ __launch_bounds__(SEARCH_THREADS) void k_search_list(
     const uint32_t* __restrict__ dq, const uint32_t* __restrict__ dc, int W, int chunk_cols,
@@ -165,9 +195,9 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_search_list(
     int16_t* __restrict__ res, int16_t* __restrict__ disp, int64_t dstep) {
     extern __shared__ uint4 lds4[];
     uint32_t* lds = (uint32_t*)lds4;
-    const int row = blockIdx.y, tid = threadIdx.x;
+    const int row = blockIdx.x, tid = threadIdx.x;
     const int count = row_count[row];
-    const int i0 = blockIdx.x * SEARCH_THREADS;
+    const int i0 = blockIdx.y * SEARCH_THREADS;
     if (i0 >= count)
         return;  // uniform over the block
     const int i = i0 + tid;
@@ -218,7 +248,7 @@ __global__ __launch_bounds__(256) void k_lr_combine(const int16_t* __restrict__ 
                                                     const int16_t* __restrict__ fwd,
                                                     const int16_t* __restrict__ rev, int max_lr_diff,
                                                     int16_t* __restrict__ disp, int64_t dstep) {
-    const int row = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    const int row = blockIdx.x, i = blockIdx.y * 256 + threadIdx.x;  // rows spread over XCDs
     if (i >= row_count[row])
         return;
     const size_t e = (size_t)row * W + i;
@@ -255,6 +285,21 @@ __global__ __launch_bounds__(256) void k_window_label(const float* __restrict__ 
             ++cnt;
         }
     label[(size_t)Y * Wq + X] = cnt >= min_valid && cnt > 0 ? s / (float)cnt : __builtin_nanf("");
+}
+
+// total of the row counts (one block)
+__global__ __launch_bounds__(256) void k_sum_counts(const int* __restrict__ row_count, int H,
+                                                    unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long part[256];
+    unsigned long long s = 0;
+    for (int r = threadIdx.x; r < H; r += 256) s += (unsigned long long)row_count[r];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = part[0];
 }
 
 // sum of the per-cell losses and label counts, added to acc[0], acc[1]

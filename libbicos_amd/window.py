@@ -57,6 +57,15 @@ def _check_params(params):
         raise Exception(f"window parameters must be a contiguous float32 tensor of {PARAM_COUNT}")
 
 
+CNN_MFMA, CNN_VALU = 0, 1
+
+
+def set_cnn_impl(impl: int) -> None:
+    """Network implementation of predict / match_windowed_device for later
+    calls: CNN_MFMA (default, matrix cores) or CNN_VALU (training kernels)."""
+    _check(_lib.bicos_hip_set_cnn_impl(int(impl)))
+
+
 # ----------------------------------------------------------------- weights
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -291,7 +300,7 @@ def window_hits(center, hw, labels):
 
 
 def evaluate(params, spec=None, scenes: int = 200, opts: Optional[WindowOptions] = None,
-             precision: int = 2, seed0: int = EVAL_SEED0, device="cuda",
+             precision: int = 3, seed0: int = EVAL_SEED0, device="cuda",
              log: Optional[Callable[[str], None]] = print):
     """Learned vs plain path on `scenes` unseen scenes: agreement (share of
     plain-valid pixels that are valid in the learned path and within 0.5 px),
@@ -353,7 +362,7 @@ def _main(argv=None):
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--scenes", type=int, default=200)
-    e.add_argument("--precision", type=int, default=2)
+    e.add_argument("--precision", type=int, default=3)
     e.add_argument("--hw-lo", type=int, default=None)
     e.add_argument("--hw-hi", type=int, default=None)
     e.add_argument("--p-conf", type=float, default=None)

@@ -86,12 +86,16 @@ def conv_bwd(x, w, dy):
     return dxp[:, 1:-1, 1:-1], dw, dy.sum(axis=(1, 2))
 
 
-def features(x0, L):
-    acts = [x0]
-    h = x0
+def features(x0, L, bf=True):
+    """activations x0, a1, a2, a3, f; with bf every one is rounded to BF16
+    (the forward pass of the HIP kernels); the backward pass treats the
+    rounding as the identity"""
+    rnd = (lambda a: bf16(a.astype(np.float32)).astype(np.float64)) if bf else (lambda a: a)
+    h = rnd(x0)
+    acts = [h]
     for l, (w, b) in enumerate(L):
         z = conv(h, w, b)
-        h = np.maximum(z, 0) if l < 3 else z
+        h = rnd(np.maximum(z, 0) if l < 3 else z)
         acts.append(h)
     return acts
 
@@ -153,10 +157,10 @@ def corr_bwd(fl, fr, dC):
 
 def loss_and_grad(framesL, framesR, label, p, in_scale, bf=True, bins=K):
     """mean cross-entropy over labelled cells and its gradient w.r.t. the flat
-    parameters (weights through their BF16 values when bf)"""
+    parameters (weights and activations through their BF16 values when bf)"""
     L = layers(p, bf)
-    aL = features(cnn_input(framesL, in_scale), L)
-    aR = features(cnn_input(framesR, in_scale), L)
+    aL = features(cnn_input(framesL, in_scale), L, bf)
+    aR = features(cnn_input(framesR, in_scale), L, bf)
     Cv = corr(aL[-1], aR[-1], bins)
     loss, cnt, dC = softmax_ce(Cv, label)
     cnt = max(cnt, 1)

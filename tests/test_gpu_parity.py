@@ -420,6 +420,61 @@ def test_precision_half_mfma_vs_fp64_oracle(gpu, oracle):
     assert (np.isfinite(plain) | ~np.isfinite(got)).mean() > 0.99
 
 
+@pytest.mark.parametrize("which", ["c1", "c0"])
+def test_precision_moments_vs_fp64_oracle(gpu, oracle, which):
+    """MOMENTS subpixel mode: disparity error of each result against the FP64
+    oracle (Precision::DOUBLE semantics, agree.cuh:35-65), measured and
+    bounded; with a corrmap asked for, the exact FP32 stage runs and the
+    result is bit-identical to it."""
+    from libbicos_amd import scene
+    if which == "c1":
+        L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=112, height=96)
+        kw = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+        okw = dict(thr=0.96, step=0.1, min_variance=2.0)
+    else:
+        L, R, _ = scene.generate(scene.C0_640x480_U8, seed=113, height=64)
+        kw = dict(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
+        okw = dict(thr=0.9, step=0.25, full=True)
+    cfg = gpu.Config(precision=gpu.Precision.Moments, **kw)
+    got = host(gpu.match_device(dev(L), dev(R), cfg)[0])
+    exp, exp_cm = oracle.match(L, R, double=True, corrmap=True, **okw)
+    st = precision_stats(got, exp_cm, exp, exp_cm)
+    print(f"MOMENTS {which} vs FP64:", st)
+    assert st["valid_both"] > 0.5 * got.size
+    # one grid step where two grid points are within the rounding bound; the
+    # 8-bit samples round coarsely, so their bound is wider
+    assert st["d_mean"] < 0.02 and st["d_over_0p1"] < (0.01 if which == "c1" else 0.08)
+    assert st["d_max"] <= 2.0
+    assert st["flip_frac"] == 0.0
+    with_cm, cm = run_match(gpu, L, R, cfg)
+    ex, ex_cm = run_match(gpu, L, R, gpu.Config(**kw))
+    assert same(with_cm, ex) and same(cm, ex_cm)
+
+
+@pytest.mark.parametrize("which", ["c1", "c0"])
+def test_precision_moments_decisions_exact(gpu, oracle, which):
+    """MOMENTS without a corrmap settles accept/reject by a bound on what the
+    rounding can change and rescores only where the bound cannot decide:
+    the valid set is exactly the exact FP32 path's, and the grid point
+    matches it for nearly every pixel."""
+    from libbicos_amd import scene
+    if which == "c1":
+        L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=114, height=96)
+        kw = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    else:
+        L, R, _ = scene.generate(scene.C0_640x480_U8, seed=115, height=64)
+        kw = dict(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
+    d = gpu.match_device(dev(L), dev(R), gpu.Config(precision=gpu.Precision.Moments, **kw))[0]
+    got = host(d)
+    ex, _ = run_match(gpu, L, R, gpu.Config(**kw))
+    assert np.array_equal(np.isfinite(got), np.isfinite(ex))
+    v = np.isfinite(ex)
+    same = (got[v] == ex[v]).mean()
+    print(f"MOMENTS {which} without corrmap: same grid point for {same:.4f} of valid pixels, "
+          f"max |d| {np.abs(got[v] - ex[v]).max():.2f}")
+    assert same > (0.99 if which == "c1" else 0.9)
+
+
 def test_precision_half_mfma_u8_full_mode(gpu, oracle):
     from libbicos_amd import scene
     L, R, _ = scene.generate(scene.C0_640x480_U8, seed=111, height=64)

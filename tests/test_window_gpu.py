@@ -186,6 +186,31 @@ def test_training_reduces_loss(W):
     assert last < 0.8 * first
 
 
+# ------------------------------------------------------ MFMA inference
+
+@pytest.mark.parametrize("bins", [80, 144])
+def test_mfma_network_matches_valu(W, bins):
+    """The inference network on the matrix cores computes the training
+    kernels' network (BF16 weights and activations, FP32 sums) up to
+    summation order: same windows almost everywhere."""
+    from libbicos_amd import scene
+    L, R, _ = W.scene_device(scene.C1_1920x1200_U16, 77, frames=4, height=96, width=640)
+    p = W.load()[0] if os.path.exists(W.DEFAULT_WEIGHTS) else W.init_params(5)
+    params = W.params_device(p)
+    opts = W.WindowOptions(bins=bins)
+    try:
+        W.set_cnn_impl(W.CNN_VALU)
+        cv, hv, pv = (host(t) for t in W.predict(L, R, params, opts))
+    finally:
+        W.set_cnn_impl(W.CNN_MFMA)
+    cm, hm, pm = (host(t) for t in W.predict(L, R, params, opts))
+    same = np.abs(cm - cv) <= 0.05
+    print(f"bins {bins}: centres equal {same.mean():.4f}, max |dp| {np.abs(pm - pv).max():.4f}")
+    assert same.mean() >= 0.98
+    assert (hm == hv).mean() >= 0.98
+    assert np.median(np.abs(pm - pv)) <= 1e-3
+
+
 # ------------------------------------------------------ windowed matching
 
 def c1_crop(W, seed, H=64, Wd=480, frames=33):
@@ -194,9 +219,10 @@ def c1_crop(W, seed, H=64, Wd=480, frames=33):
     return L, R
 
 
+@pytest.mark.parametrize("prec", ["single", "moments"])
 @pytest.mark.parametrize("variant", ["nodup", "cons"])
 @pytest.mark.parametrize("subpix", [True, False])
-def test_full_width_window_equals_plain(W, variant, subpix):
+def test_full_width_window_equals_plain(W, variant, subpix, prec):
     """A window spanning the row makes the windowed search the full search:
     the learned path must then reproduce the plain path bit for bit."""
     import libbicos_amd as B
@@ -204,6 +230,8 @@ def test_full_width_window_equals_plain(W, variant, subpix):
     kw = dict(C1_CFG)
     if not subpix:
         kw.pop("subpixel_step")
+    if prec == "moments":
+        kw["precision"] = B.Precision.Moments
     if variant == "cons":
         kw["variant"] = B.Variant.Consistency(max_lr_diff=1, no_dupes=False)
     cfg = B.Config(**kw)
@@ -215,34 +243,56 @@ def test_full_width_window_equals_plain(W, variant, subpix):
     assert np.array_equal(a, b, equal_nan=True)
     assert np.array_equal(host(c_plain), host(c_win), equal_nan=True)
     assert st["pixels"] == a.size
+    # without a corrmap (the MOMENTS stage proper)
+    a = host(B.match_device(L, R, cfg)[0])
+    b = host(W.match_windowed_device(L, R, cfg, params, opts)[0])
+    assert np.array_equal(a, b, equal_nan=True)
 
 
+@pytest.mark.parametrize("impl", ["mfma", "valu", "moments"])
 @pytest.mark.parametrize("variant", ["nodup", "cons"])
 @pytest.mark.parametrize("subpix", [True, False])
 @pytest.mark.parametrize("hw", [0, 3])
-def test_fallback_invariant_untrained(W, variant, subpix, hw):
+def test_fallback_invariant_untrained(W, variant, subpix, hw, impl):
     """With any network, every pixel the learned path rejects was sent
     through the exact full search and NCC stage: the plain path rejects it
     too. Pixels the learned path accepts either match the plain result or
-    pass the NCC threshold at another disparity."""
+    pass the NCC threshold at another disparity. Both row-search kernels
+    (matrix cores, vector ALU) serve the listed pixels."""
     import libbicos_amd as B
+    if impl == "moments" and not subpix:
+        pytest.skip("the MOMENTS precision concerns the subpixel stage")
+    B.set_search_impl(B.SEARCH_VALU if impl == "valu" else B.SEARCH_MFMA)
+    try:
+        _fallback_invariant(W, B, variant, subpix, hw, impl == "moments")
+    finally:
+        B.set_search_impl(B.SEARCH_MFMA)
+
+
+def _fallback_invariant(W, B, variant, subpix, hw, moments):
     L, R = c1_crop(W, 32 + hw)
     kw = dict(C1_CFG)
     if not subpix:
         kw.pop("subpixel_step")
+    if moments:
+        kw["precision"] = B.Precision.Moments
     if variant == "cons":
         kw["variant"] = B.Variant.Consistency(max_lr_diff=1, no_dupes=False)
     cfg = B.Config(**kw)
     params = W.params_device(W.init_params(2))
     opts = W.WindowOptions(hw_lo=hw, hw_hi=hw)
     d_plain, _ = B.match_device(L, R, cfg)
-    d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, want_corrmap=True, stats=True)
+    if moments:  # no corrmap: the MOMENTS stage proper
+        d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, stats=True)
+    else:
+        d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, want_corrmap=True, stats=True)
     a, b = host(d_plain), host(d_win)
     inv_a = np.isnan(a) if subpix else a == -32768
     inv_b = np.isnan(b) if subpix else b == -32768
     assert not (inv_b & ~inv_a).any(), "learned path rejected a pixel the plain path accepts"
-    cw = host(c_win)
-    assert (cw[~inv_b] >= 0.96 - 1e-6).all()
+    if c_win is not None:
+        cw = host(c_win)
+        assert (cw[~inv_b] >= 0.96 - 1e-6).all()
     assert st["fallback_pixels"] > 0
     print(f"hw {hw}: fallback {st['fallback_pixels'] / a.size:.3f}, "
           f"agree {(a == b)[~inv_a].mean():.4f}")
