@@ -5,10 +5,12 @@
 A step is one BICOS::match call over one synthetic 1920x1200, 33-frame uint16
 stereo pair (LIMITED descriptors, min_variance 2.0, nxcorr_threshold 0.96,
 subpixel_step 0.1, NoDuplicates) whose stacks are already resident in HBM.
-Prints one JSON line: throughput in matched Mpixel/s, the roofline position of
-the dominant kernel (per-kernel times from HIP events recorded by the library
-on the launch stream during the timed steps), and the CPU oracle timed on the
-host on the same workload.
+Prints one JSON line: throughput in matched Mpixel/s (`value`: the plain,
+bit-exact path), the roofline position of its dominant kernel (per-kernel times
+from HIP events recorded by the library on the launch stream during the timed
+steps), the learned-window path on the same pair (throughput, ratio to the
+plain path, agreement with it, fallback rate; with and without the full-row
+fallback), and the CPU oracle timed on the host on the same workload.
 """
 from __future__ import annotations
 
@@ -36,6 +38,10 @@ def parse_args():
     p.add_argument("--seed", type=int, default=0x600DF00D)
     p.add_argument("--search", choices=("mfma", "valu"), default="mfma",
                    help="row-search kernel: int8 matrix cores (default) or vector ALU")
+    p.add_argument("--workload", choices=("c1", "stress"), default="c1",
+                   help="c1: the 1920x1200x33 benchmark line (default); stress: the "
+                        "4096x3000x64 Consistency workload, plain and learned, with peak "
+                        "device memory")
     return p.parse_args()
 
 
@@ -89,12 +95,124 @@ def cpu_baseline(L, R, cfg_kw):
                       f"{threads} threads, oracle/bicos_oracle.cpp)"}
 
 
+def learned_path(B, dL, dR, plain_disp, args, W, H, plain_ms):
+    """The learned-window path (MOMENTS subpixel stage, committed weights)
+    on the same resident pair: timed like the plain path, then compared with
+    the plain disparity (share of plain-valid pixels valid and within 0.5 px)."""
+    import torch
+    from libbicos_amd import window as Wn
+    if not os.path.exists(Wn.DEFAULT_WEIGHTS):
+        return None
+    params = Wn.params_device(Wn.load()[0])
+    cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
+                   precision=B.Precision.Moments)
+    stream = torch.cuda.current_stream()
+    a = plain_disp
+    va = torch.isfinite(a)
+    res = {"weights": os.path.relpath(Wn.DEFAULT_WEIGHTS, ROOT), "precision": "MOMENTS",
+           "agreement_rule": "plain-valid pixels valid and within 0.5 px of plain"}
+    for name, fb in (("fallback_all", True), ("fallback_none", False)):
+        opts = Wn.WindowOptions(fallback=fb)
+        out, _, st = Wn.match_windowed_device(dL, dR, cfg, params, opts, stats=True, stream=stream)
+        for _ in range(args.warmup):
+            out, _, _ = Wn.match_windowed_device(dL, dR, cfg, params, opts, disparity=out, stream=stream)
+        B.timing_enable(True)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            out, _, _ = Wn.match_windowed_device(dL, dR, cfg, params, opts, disparity=out, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        kst = B.timing_read()
+        B.timing_enable(False)
+        ms = e0.elapsed_time(e1) / args.steps
+        b = out
+        ok = torch.isfinite(b) & ((b - a).abs() <= 0.5)
+        res[name] = {
+            "value": W * H / (ms * 1e-3) / 1e6, "unit": "Mpixel/s", "ms_per_step": ms,
+            "vs_plain": plain_ms / ms,
+            "agreement": (ok & va).sum().item() / max(1, va.sum().item()),
+            "learned_valid_plain_invalid": (torch.isfinite(b) & ~va).float().mean().item(),
+            "fallback_rate": st["fallback_pixels"] / st["pixels"],
+            "kernels_ms": {k: round(t / args.steps, 4) for k, (c, t) in sorted(kst.items(), key=lambda kv: -kv[1][1])},
+        }
+    return res
+
+
+def stress(args):
+    """4096x3000, 64 frames/side u16 (256-bit descriptors), Consistency
+    (max_lr_diff 1), subpixel step 0.05, threshold 0.96, min_variance 2.0:
+    plain and learned paths, device memory in use after each (the library
+    keeps its workspace), per-kernel times and algorithmic GB/s."""
+    import torch
+    sys.path.insert(0, ROOT)
+    import libbicos_amd as B
+    from libbicos_amd import scene, window as Wn
+    spec = scene.C4_4096x3000_U16
+    W, H, n = spec.width, spec.height, spec.frames
+    free0, total = torch.cuda.mem_get_info()
+    L, R, _ = Wn.scene_device(spec, args.seed)
+    torch.cuda.synchronize()
+    B.release_workspace()
+    free1, _ = torch.cuda.mem_get_info()
+    kw = dict(nxcorr_threshold=0.96, subpixel_step=0.05, min_variance=2.0,
+              variant=B.Variant.Consistency(max_lr_diff=1, no_dupes=False))
+    out = {"workload": f"BICOS::match {W}x{H}x{n} u16, LIMITED (256-bit), Consistency "
+                       "max_lr_diff 1, subpixel_step 0.05, nxcorr_threshold 0.96, min_variance 2.0",
+           "stacks_bytes": free0 - free1}
+    runs = [("plain", None, B.Config(**kw)),
+            ("learned_fallback_all", True, B.Config(precision=B.Precision.Moments, **kw)),
+            ("learned_fallback_none", False, B.Config(precision=B.Precision.Moments, **kw))]
+    params = Wn.params_device(Wn.load()[0]) if os.path.exists(Wn.DEFAULT_WEIGHTS) else None
+    plain = None
+    for name, fb, cfg in runs:
+        if fb is not None and params is None:
+            continue
+        opts = Wn.WindowOptions(bins=144, fallback=fb) if fb is not None else None
+        call = (lambda: B.match_device(L, R, cfg)[0]) if fb is None else \
+            (lambda: Wn.match_windowed_device(L, R, cfg, params, opts)[0])
+        d = call()
+        torch.cuda.synchronize()
+        used = total - torch.cuda.mem_get_info()[0]
+        B.timing_enable(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        steps = max(1, min(args.steps, 5))
+        e0.record()
+        for _ in range(steps):
+            d = call()
+        e1.record()
+        torch.cuda.synchronize()
+        kst = B.timing_read()
+        B.timing_enable(False)
+        ms = e0.elapsed_time(e1) / steps
+        rec = {"ms": ms, "Mpixel/s": W * H / ms / 1e3, "device_memory_in_use_bytes": used,
+               "valid_fraction": torch.isfinite(d).float().mean().item(),
+               "kernels_ms": {k: round(t / steps, 4) for k, (c, t) in sorted(kst.items(), key=lambda kv: -kv[1][1])}}
+        for k, (c, t) in kst.items():
+            if k == "transform":
+                rec["transform_GBps"] = W * H * 2 * (2 * n + 32) / (t / steps * 1e-3) / 1e9
+        if fb is None:
+            plain = d
+        else:
+            va = torch.isfinite(plain)
+            ok = torch.isfinite(d) & ((d - plain).abs() <= 0.5)
+            rec["agreement"] = (ok & va).sum().item() / max(1, va.sum().item())
+            rec["vs_plain"] = out["plain"]["ms"] / ms
+        out[name] = rec
+        B.release_workspace()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md on row-band sharding)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "stress":
+        stress(args)
+        return
 
     import numpy as np
     import torch
@@ -144,6 +262,7 @@ def main():
 
     d = disp.cpu().numpy()
     valid = float(np.isfinite(d).mean())
+    learned = learned_path(B, dL, dR, disp, args, W, H, ms_per_step)
 
     out = {
         "metric": METRIC,
@@ -165,6 +284,7 @@ def main():
                    "valid_fraction": round(valid, 4), "wall_s_timed": round(wall, 4)},
         "roofline": roof,
         "kernels": kernels,
+        "learned": learned,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(L, R, dict(thr=0.96, step=0.1, min_variance=2.0,

@@ -69,26 +69,83 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     const int d = disp[(size_t)row * dstep + col];
     const int col1 = col - d;
     const bool valid_d = d != INVALID16 && 0 <= col1 && col1 < W;
+    const bool interior = valid_d && col1 != 0 && col1 != W - 1;
+    const int c1 = interior ? col1 : 1;  // in range for the loads of other lanes
     float res = NAN;
     uint8_t st = 0;
-    // left variance as the reference forms it (also for pixels without a
-    // match: the windowed path's fallback skips hopeless pixels)
-    float mean0 = 0.0f;
-#pragma unroll 3
-    for (int t = 0; t < NMAX; ++t)
-        if (n.has(t)) mean0 += (float)load_sample_cached<TIn>(f0, t, row, col);
-    mean0 /= (float)n.get();
-    float var0 = 0.0f;
-#pragma unroll 3
+
+    // One pass: moments of the left samples (every pixel: its variance
+    // decides the fallback; FP64, exact for integer samples) and, for
+    // interior matches, of the three right columns, with the reach of the
+    // parabolas. The cross moments are FP32 sums of samples shifted by the
+    // first frame's (products below 2^32; relative error < 1e-5 of the
+    // centred moments, covered by the bound's slack below).
+    double su = 0, suu = 0;
+    float s0 = 0, s1 = 0, s2 = 0, su0 = 0, su1 = 0, su2 = 0;
+    float s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0, su_f = 0;
+    float ymin = 3.0e38f, ymax = -3.0e38f;
+    const float u_ref = (float)load_sample_cached<TIn>(f0, 0, row, col);
+    float y_ref0 = 0.0f, y_ref1 = 0.0f, y_ref2 = 0.0f;
+    if (interior) {
+        y_ref0 = (float)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
+        y_ref1 = (float)load_sample_cached<TIn>(f1, 0, row, c1);
+        y_ref2 = (float)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
+    }
+    // 11 frames per round: 44 loads in flight per lane
+#pragma unroll 11
     for (int t = 0; t < NMAX; ++t)
         if (n.has(t)) {
-            const float c = (float)load_sample_cached<TIn>(f0, t, row, col) - mean0;
-            var0 = __builtin_fmaf(c, c, var0);
+            const float uf = (float)load_sample_cached<TIn>(f0, t, row, col);
+            const double u = uf;
+            su += u;
+            suu = __builtin_fma(u, u, suu);
+            if (interior) {
+                const float f0y = (float)load_sample_cached<TIn>(f1, t, row, c1 - 1);
+                const float f1y = (float)load_sample_cached<TIn>(f1, t, row, c1);
+                const float f2y = (float)load_sample_cached<TIn>(f1, t, row, c1 + 1);
+                // reach of the frame's parabola on [-1, 1]: the end values and,
+                // where it is convex (concave) with the vertex inside, the
+                // vertex, which lies above (below) y1 -/+ |y2 - y0| / 4
+                const float r4 = 0.25f * fabsf(f2y - f0y);
+                ymin = fminf(ymin, fminf(fminf(f0y, f2y), f1y - r4));
+                ymax = fmaxf(ymax, fmaxf(fmaxf(f0y, f2y), f1y + r4));
+                const float us = uf - u_ref, y0 = f0y - y_ref0, y1 = f1y - y_ref1, y2 = f2y - y_ref2;
+                su_f += us;
+                s0 += y0; s1 += y1; s2 += y2;
+                su0 = __builtin_fmaf(us, y0, su0); su1 = __builtin_fmaf(us, y1, su1);
+                su2 = __builtin_fmaf(us, y2, su2);
+                s00 = __builtin_fmaf(y0, y0, s00); s01 = __builtin_fmaf(y0, y1, s01);
+                s02 = __builtin_fmaf(y0, y2, s02); s11 = __builtin_fmaf(y1, y1, s11);
+                s12 = __builtin_fmaf(y1, y2, s12); s22 = __builtin_fmaf(y2, y2, s22);
+            }
         }
-    const bool lowvar = mv && var0 < minvar;
+    const double inv = 1.0 / (double)n.get();
+    const double var0d = suu - su * su * inv;
+    // The reference tests its FP32 two-pass variance against min_variance
+    // (agree.hpp:158); the exact value decides unless it is within the FP32
+    // error of the threshold, where the two-pass value is formed.
+    bool lowvar = false;
+    if (mv) {
+        const double mvd = (double)minvar, slack = 1e-4 * var0d + 1.0;
+        if (var0d < mvd - slack) {
+            lowvar = true;
+        } else if (var0d <= mvd + slack) {
+            float mean0 = 0.0f;
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) mean0 += (float)load_sample_cached<TIn>(f0, t, row, col);
+            mean0 /= (float)n.get();
+            float var0 = 0.0f;
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) {
+                    const float c = (float)load_sample_cached<TIn>(f0, t, row, col) - mean0;
+                    var0 = __builtin_fmaf(c, c, var0);
+                }
+            lowvar = var0 < minvar;
+        }
+    }
     if (lowvar && thr > -1.0f) st |= ST_HOPELESS;
     if (valid_d) {
-        if (col1 == 0 || col1 == W - 1) {
+        if (!interior) {
             const float corr = nxcorr_stream<TIn>(f0, row, col, [&](int t) {
                 return (float)load_sample_cached<TIn>(f1, t, row, col1);
             }, n, mv, minvar);
@@ -96,39 +153,11 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
         } else if (lowvar) {
             if (!(-1.0f < thr)) res = (float)d;  // no grid point is scored (agree.hpp:158)
         } else {
-            // exact moments (integer samples, FP64); reach of the parabolas
-            double su = 0, s0 = 0, s1 = 0, s2 = 0, su0 = 0, su1 = 0, su2 = 0;
-            double s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0;
-            float ymin = 3.0e38f, ymax = -3.0e38f;
-#pragma unroll 3
-            for (int t = 0; t < NMAX; ++t)
-                if (n.has(t)) {
-                    const float fu = (float)load_sample_cached<TIn>(f0, t, row, col);
-                    const float f0y = (float)load_sample_cached<TIn>(f1, t, row, col1 - 1);
-                    const float f1y = (float)load_sample_cached<TIn>(f1, t, row, col1);
-                    const float f2y = (float)load_sample_cached<TIn>(f1, t, row, col1 + 1);
-                    // range of the frame's parabola on [-1, 1]: the end values
-                    // and, when the vertex x* = -b / 2a lies inside, its value
-                    // c - b^2 / 4a (here with a2 = 2a, b2 = 2b)
-                    const float a2 = __builtin_fmaf(-2.0f, f1y, f0y + f2y), b2 = f2y - f0y;
-                    const bool inside = fabsf(b2) < fabsf(2.0f * a2);
-                    const float vtx = __builtin_fmaf(-b2 * b2, __builtin_amdgcn_rcpf(8.0f * a2), f1y);
-                    const float ends_lo = fminf(f0y, f2y), ends_hi = fmaxf(f0y, f2y);
-                    ymin = fminf(ymin, inside ? fminf(ends_lo, vtx) : ends_lo);
-                    ymax = fmaxf(ymax, inside ? fmaxf(ends_hi, vtx) : ends_hi);
-                    const double u = fu, y0 = f0y, y1 = f1y, y2 = f2y;
-                    su += u;
-                    s0 += y0; s1 += y1; s2 += y2;
-                    su0 = __builtin_fma(u, y0, su0); su1 = __builtin_fma(u, y1, su1);
-                    su2 = __builtin_fma(u, y2, su2);
-                    s00 = __builtin_fma(y0, y0, s00); s01 = __builtin_fma(y0, y1, s01);
-                    s02 = __builtin_fma(y0, y2, s02); s11 = __builtin_fma(y1, y1, s11);
-                    s12 = __builtin_fma(y1, y2, s12); s22 = __builtin_fma(y2, y2, s22);
-                }
-            const double inv = 1.0 / (double)n.get();
-            const double u0 = su0 - su * s0 * inv, u1 = su1 - su * s1 * inv, u2 = su2 - su * s2 * inv;
-            const double g00 = s00 - s0 * s0 * inv, g01 = s01 - s0 * s1 * inv, g02 = s02 - s0 * s2 * inv;
-            const double g11 = s11 - s1 * s1 * inv, g12 = s12 - s1 * s2 * inv, g22 = s22 - s2 * s2 * inv;
+            // centred moments (FP64 from the FP32 sums)
+            const double su_s = su_f, d0 = s0, d1 = s1, d2 = s2;
+            const double u0 = su0 - su_s * d0 * inv, u1 = su1 - su_s * d1 * inv, u2 = su2 - su_s * d2 * inv;
+            const double g00 = s00 - d0 * d0 * inv, g01 = s01 - d0 * d1 * inv, g02 = s02 - d0 * d2 * inv;
+            const double g11 = s11 - d1 * d1 * inv, g12 = s12 - d1 * d2 * inv, g22 = s22 - d2 * d2 * inv;
             const float ua = (float)(0.5 * (u0 - 2.0 * u1 + u2)), ub = (float)(0.5 * (u2 - u0)), uc = (float)u1;
             const float aa = (float)(0.25 * (g00 + 4.0 * g11 + g22 - 4.0 * g01 + 2.0 * g02 - 4.0 * g12));
             const float ab = (float)(0.25 * (g22 - g00 - 2.0 * g12 + 2.0 * g01));
@@ -139,25 +168,29 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             // Bound on what the left-out rounding can change. If every
             // frame's parabola stays inside [1, max(TIn) - 1] (with 0.1 for
             // the FP32 evaluation and the approximate vertex), the cast of
-            // agree.hpp:164 is the identity
-            // and every interpolated sample moves by at most 0.51 (0.5 plus
-            // the FP32 evaluation error): |e| <= delta = 0.51 sqrt(n). Then
-            // with s = sqrt(var1), covariance and norm move by at most
-            // sqrt(var0) delta and delta, the correlation by at most
-            // 2 delta / (s - delta) (+1e-5 for the FP32 arithmetic of both
-            // forms), and var1 itself by at most 2 s delta + delta^2.
+            // agree.hpp:164 is the identity and every interpolated sample
+            // moves by at most 0.51 (0.5 plus the FP32 evaluation error):
+            // |e| <= delta = 0.51 sqrt(n). Then with s = sqrt(var1),
+            // covariance and norm move by at most sqrt(var0) delta and delta,
+            // the correlation by at most 2 delta / (s - delta) (+1e-4 for the
+            // FP32 moments and arithmetic of both forms and the exact versus
+            // two-pass left variance), and var1 itself by at most
+            // 2 s delta + delta^2 (+1e-4 var1).
             constexpr float VMAX = (float)(TIn)~(TIn)0;
             const float delta = 0.51f * __builtin_sqrtf((float)n.get());
             float best_x = 0.0f, best = -1.0f, eps_best = 2.0f;
             float upper = -1.0f;  // max over x of closed form + bound
             bool settled = ymin >= 1.1f && ymax <= VMAX - 1.1f;  // (NaN-safe: NaN fails)
-            const float rv0 = __builtin_amdgcn_rsqf(var0);
+            const float rv0 = __builtin_amdgcn_rsqf((float)var0d);
+            const auto bound = [&](float sv) {
+                return 2.000001f * delta * __builtin_amdgcn_rcpf(sv - delta) + 1e-4f;
+            };
             for (float x = -1.0f; x <= 1.0f; x += step) {
                 const float cov = (ua * x + ub) * x + uc;
                 const float v1 = (((aa * x + 2.0f * ab) * x + (bb + 2.0f * ac)) * x + 2.0f * bc) * x + cc;
                 const float r1 = __builtin_amdgcn_rsqf(v1);
                 const float sv = v1 * r1;  // sqrt(var1)
-                const float margin = 2.0f * sv * delta + delta * delta + 1.0f;
+                const float margin = 2.0f * sv * delta + delta * delta + 1e-4f * v1 + 1.0f;
                 float nxc = -1.0f, eps = 0.0f;
                 if (mv && v1 < minvar) {
                     // below the variance test: also exactly when below by the margin
@@ -166,7 +199,7 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     if (mv && v1 < minvar + margin) settled = false;
                     if (!(sv > 2.0f * delta)) settled = false;  // no useful bound here
                     nxc = cov * r1 * rv0;
-                    eps = 2.0f * delta / (sv - delta) + 1e-5f;
+                    eps = bound(sv);
                 }
                 upper = fmaxf(upper, nxc + eps);
                 if (best < nxc) {
@@ -186,15 +219,15 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     const float v1 = (((aa * x + 2.0f * ab) * x + (bb + 2.0f * ac)) * x + 2.0f * bc) * x + cc;
                     const float r1 = __builtin_amdgcn_rsqf(v1);
                     if (mv && v1 < minvar) continue;  // settled: exactly -1 there
-                    far = fmaxf(far, cov * r1 * rv0 + 2.0f * delta / (v1 * r1 - delta) + 1e-5f);
+                    far = fmaxf(far, cov * r1 * rv0 + bound(v1 * r1));
                 }
                 if (far < best - eps_best)
                     res = (float)d - best_x;
                 else
                     st |= ST_PENDING;
-            }
-            else if (!(settled && upper < thr))
+            } else if (!(settled && upper < thr)) {
                 st |= ST_PENDING;
+            }
         }
     }
     out[(size_t)row * ostep + col] = res;

@@ -428,6 +428,15 @@ int run_wide_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack&
     });
 }
 
+int run_moments_only(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                     int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                     float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
+                     hipStream_t s) {
+    return depth == BICOS_8U
+        ? run_subpixel_moments_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s)
+        : run_subpixel_moments_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s);
+}
+
 // the pending pixels of the MOMENTS status, through the exact stage
 int run_pending_exact(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                       int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,

@@ -133,6 +133,12 @@ struct MomentsScratch {
 int run_status_list(const uint8_t* status, const float* res, int64_t rstep, int mask, int W, int H,
                     int16_t* list, int* row_count, hipStream_t s);
 
+// MOMENTS closed-form stage alone (status written; pending pixels left NaN)
+int run_moments_only(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                     int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                     float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
+                     hipStream_t s);
+
 // MOMENTS stage of listed pixels (row r: row_count[r] columns at
 // list + r * W), writing their status, then the exact stage on those left
 // pending

@@ -181,6 +181,43 @@ __device__ __forceinline__ void load_px4(const Stack& f, int t, int row, int col
     }
 }
 
+// 4 adjacent raw samples of frame t (zero beyond the row end)
+template<typename TIn>
+__device__ __forceinline__ void load_raw4(const Stack& f, int t, int row, int col, int W,
+                                          uint32_t (&r)[TPX * sizeof(TIn) / 4]) {
+    const TIn* p = frame_row<TIn>(f, t, row) + col;
+    if (col + TPX <= W) {
+        if constexpr (sizeof(TIn) == 2) {
+            const uint2 u = *(const uint2*)p;
+            r[0] = u.x;
+            r[1] = u.y;
+        } else {
+            r[0] = *(const uint32_t*)p;
+        }
+    } else {
+        TIn v[TPX];
+#pragma unroll
+        for (int k = 0; k < TPX; ++k) v[k] = col + k < W ? p[k] : (TIn)0;
+        if constexpr (sizeof(TIn) == 2) {
+            r[0] = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+            r[1] = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        } else {
+            r[0] = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+        }
+    }
+}
+
+template<typename TIn>
+__device__ __forceinline__ void unpack4(const uint32_t (&r)[TPX * sizeof(TIn) / 4], float (&v)[TPX]) {
+    if constexpr (sizeof(TIn) == 2) {
+        v[0] = (float)(r[0] & 0xFFFFu); v[1] = (float)(r[0] >> 16);
+        v[2] = (float)(r[1] & 0xFFFFu); v[3] = (float)(r[1] >> 16);
+    } else {
+        v[0] = (float)(r[0] & 0xFFu); v[1] = (float)((r[0] >> 8) & 0xFFu);
+        v[2] = (float)((r[0] >> 16) & 0xFFu); v[3] = (float)(r[0] >> 24);
+    }
+}
+
 // shift the comparison x < y into the low end of g
 __device__ __forceinline__ uint32_t shin(uint32_t g, float x, float y) {
     return __builtin_amdgcn_alignbit(g, __float_as_uint(x - y), 31);
@@ -215,17 +252,20 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
     const Stack& f = blockIdx.z ? f1 : f0;
     uint32_t* __restrict__ out = blockIdx.z ? out1 : out0;
 
-    float p[NMAX][TPX];
+    // All frames are loaded once and held as raw samples (packed, two per
+    // register for 16-bit input): the mean first, then the comparisons over
+    // a sliding window of three unpacked frames. Half the registers of
+    // holding floats, so more waves hide the memory latency.
+    uint32_t raw[NMAX][TPX * sizeof(TIn) / 4];
     float av[TPX] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int t = 0; t < NMAX; ++t) {
         if (t < n) {
-            load_px4<TIn>(f, t, row, col, W, p[t]);
+            load_raw4<TIn>(f, t, row, col, W, raw[t]);
+            float v[TPX];
+            unpack4<TIn>(raw[t], v);
 #pragma unroll
-            for (int k = 0; k < TPX; ++k) av[k] += p[t][k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < TPX; ++k) p[t][k] = 0.0f;
+            for (int k = 0; k < TPX; ++k) av[k] += v[k];
         }
     }
 #pragma unroll
@@ -237,14 +277,18 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
 #pragma unroll
         for (int j = 0; j < WORDS; ++j) w[k][j] = 0u;
 
+    float pa[TPX], pb[TPX], pc[TPX];  // frames t, t + 1, t + 2
+    unpack4<TIn>(raw[0], pa);
+    unpack4<TIn>(raw[1], pb);
     float prev0[TPX], prev1[TPX];
 #pragma unroll
     for (int t = 0; t < NMAX - 2; ++t) {
         if (t < n - 2) {
             const int pos = t < 2 ? 3 * t : 4 * t - 2;
+            unpack4<TIn>(raw[t + 2], pc);
 #pragma unroll
             for (int k = 0; k < TPX; ++k) {
-                const float a = p[t][k], b = p[t + 1][k], c = p[t + 2][k];
+                const float a = pa[k], b = pb[k], c = pc[k];
                 const float cur = a + b;
                 uint32_t g = 0;
                 if (t >= 2) g = shin(g, (t & 1) ? prev1[k] : prev0[k], cur);
@@ -254,20 +298,21 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
                 put_group(w[k], pos, t >= 2 ? 4 : 3, g);
                 if (t & 1) prev1[k] = cur; else prev0[k] = cur;
             }
+#pragma unroll
+            for (int k = 0; k < TPX; ++k) {
+                pa[k] = pb[k];
+                pb[k] = pc[k];
+            }
         }
     }
 
-    // closing bits: a<b, a<av, b<av, S(n-4) < S(n-2) (sentinel -1 for n < 4)
+    // closing bits: a<b, a<av, b<av, S(n-4) < S(n-2) (sentinel -1 for n < 4);
+    // pa, pb now hold frames n - 2, n - 1
     const int P = n >= 4 ? 4 * n - 10 : 3 * (n - 2);
     const int wi = P >> 5, sh = P & 31;
 #pragma unroll
     for (int k = 0; k < TPX; ++k) {
-        float a = 0.0f, b = 0.0f;
-#pragma unroll
-        for (int t = 0; t < NMAX; ++t) {
-            if (t == n - 2) a = p[t][k];
-            if (t == n - 1) b = p[t][k];
-        }
+        const float a = pa[k], b = pb[k];
         const float prevf = n < 4 ? -1.0f : (((n - 2) & 1) ? prev1[k] : prev0[k]);
         uint32_t g = 0;
         g = shin(g, prevf, a + b);

@@ -589,7 +589,12 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         const int64_t cstep = cm ? corrmap->step / elem_size(corrmap->type) : 0;
         float* outf = subpix ? (float*)disparity->data : nullptr;
         const int64_t ostep = subpix ? disparity->step / 4 : 0;
-        if (subpix)
+        if (moments && !cm && fallback)
+            // the pixels it leaves pending join the fallback set (whose
+            // results then equal the plain path's) and are decided there
+            r = run_moments_only(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                                 cfg->subpixel_step, cfg->has_min_variance, mv_n, outf, ostep, &ms, s);
+        else if (subpix)
             r = run_subpixel(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                              cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision, outf,
                              ostep, cm, cstep, s, moments ? &ms : nullptr);
