@@ -195,11 +195,18 @@ typedef struct bicos_window_options {
     int32_t hw_lo;         /* window half-width (px) of confident cells */
     int32_t hw_hi;         /* ... and of the others */
     int32_t bins;          /* correlation offsets: disparities 4 * (-8 .. bins - 9) px */
-    int32_t fallback;      /* 1: full-row search of the pixels whose windowed match the
-                              NCC stage rejects (those whose left variance rejects every
-                              match excepted); a pixel the windowed path then rejects is
-                              one the plain path rejects too. 0: no fallback */
+    int32_t fallback;      /* BICOS_FALLBACK_*: which rejected pixels get the full-row search */
 } bicos_window_options;
+
+/* Full-row search of the pixels whose windowed match the NCC stage rejects
+ * (those whose left variance already rejects every match excepted):
+ *   NONE    none;
+ *   ALL     all of them: a pixel the windowed path rejects is then one the
+ *           plain path rejects too;
+ *   UNSURE  those in cells the network was unsure about (peak probability
+ *           below p_conf: the cells given the hw_hi window), and the pixels
+ *           the MOMENTS stage leaves undecided. */
+enum { BICOS_FALLBACK_NONE = 0, BICOS_FALLBACK_ALL = 1, BICOS_FALLBACK_UNSURE = 2 };
 
 typedef struct bicos_window_stats {
     int64_t pixels;            /* pixels matched */

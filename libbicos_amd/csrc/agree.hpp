@@ -107,20 +107,20 @@ __global__ __launch_bounds__(256) void k_agree(int16_t* __restrict__ disp, int64
                                         minvar, corrmap, cstep);
 }
 
-// k_agree over listed pixels only: row r = blockIdx.x (rows spread over the
-// XCDs) holds row_count[r] columns at list + r * W
+// k_agree over listed pixels only (dense numbering of the lists, grid-stride)
 template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree_list(int16_t* __restrict__ disp, int64_t dstep,
                                                     const Stack f0, const Stack f1, int n_rt,
                                                     int W, float thr, int has_minvar, float minvar,
                                                     TPrec* __restrict__ corrmap, int64_t cstep,
-                                                    const int16_t* __restrict__ list,
-                                                    const int* __restrict__ row_count) {
-    const int row = blockIdx.x, i = blockIdx.y * 256 + threadIdx.x;
-    if (i >= row_count[row])
-        return;
-    agree_px<TIn, TPrec, NMAX, N_EXACT>(row, list[(size_t)row * W + i], disp, dstep, f0, f1, n_rt,
-                                        W, thr, has_minvar, minvar, corrmap, cstep);
+                                                    const RowLists l) {
+    const int total = l.row_start[l.H];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+        int row, col;
+        list_entry(l, i, row, col);
+        agree_px<TIn, TPrec, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, has_minvar,
+                                            minvar, corrmap, cstep);
+    }
 }
 
 // The grid scan of agree_subpixel (agree.hpp:160-172): NCC of the left pixel
@@ -285,14 +285,14 @@ template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree_subpixel_list(
     const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1,
     int n_rt, int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
-    int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep, const int16_t* __restrict__ list,
-    const int* __restrict__ row_count) {
-    const int row = blockIdx.x, i = blockIdx.y * 256 + threadIdx.x;
-    if (i >= row_count[row])
-        return;
-    agree_subpixel_px<TIn, TPrec, NMAX, N_EXACT>(row, list[(size_t)row * W + i], disp, dstep, f0, f1,
-                                                 n_rt, W, thr, step, has_minvar, minvar, out, ostep,
-                                                 corrmap, cstep);
+    int64_t ostep, TPrec* __restrict__ corrmap, int64_t cstep, const RowLists l) {
+    const int total = l.row_start[l.H];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+        int row, col;
+        list_entry(l, i, row, col);
+        agree_subpixel_px<TIn, TPrec, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, step,
+                                                     has_minvar, minvar, out, ostep, corrmap, cstep);
+    }
 }
 
 }  // namespace bicos_hip

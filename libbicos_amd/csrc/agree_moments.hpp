@@ -15,10 +15,12 @@
 // (which can differ from the exact scan's where two grid points within half
 // a pixel of each other lie within the bound; a farther rival sends the
 // pixel to the exact stage).
-// Every other pixel is marked pending and gets the exact FP32 subpixel
-// stage (k_agree_subpixel_list) afterwards, so accept/reject is the
-// reference's for every pixel. Samples are streamed from memory (twice
-// where the reference's two-pass sums need them), not held in registers.
+// Every other pixel is pending: before the kernel writes its results, the
+// wave runs the exact FP32 subpixel stage on its pending pixels
+// (exact_pending_wave), one at a time with all 64 lanes, while their samples
+// are still in the L2 cache; so accept/reject is the reference's for every
+// pixel. Samples are streamed from memory (twice where the reference's
+// two-pass sums need them), not held in registers.
 #pragma once
 
 #include "agree.hpp"
@@ -56,21 +58,168 @@ __device__ __forceinline__ float nxcorr_stream(const Stack& f0, int row, int col
 }
 
 // Status bits of the MOMENTS stage, per pixel
-constexpr uint8_t ST_PENDING = 1;   // the exact stage decides this pixel
+constexpr uint8_t ST_PENDING = 1;   // the exact stage decides this pixel (cleared by it)
 constexpr uint8_t ST_HOPELESS = 2;  // left variance rejects every match
 
+// Pending pixels the exact stage takes per round of one wave, and the LDS
+// it stages their samples in: per wave EXACT_GROUPS * NMAX entries of
+// (left sample, parabola coefficients a, b, c) of frame t.
+constexpr int EXACT_GROUPS = 4;
+
+// The exact subpixel stage (agree.hpp:94-185 arithmetic, bit-identical to
+// k_agree_subpixel) for the interior matches of this wave's lanes in `pend`
+// (row, col, d: each lane's own pixel), run by the whole wave. P =
+// min(npts, 64) lanes score one pixel (lane k0 of its group takes the grid
+// points k0, k0 + P, ..., x formed by the reference's running float sum),
+// so up to min(64 / P, EXACT_GROUPS) pending pixels go per round. Their
+// samples come from memory once per round, all loads in flight together,
+// into the wave's LDS slice `xs`; both of the reference's passes read them
+// there. Each group reduces to the reference's first strict maximum, and
+// the owner lane gets the result in res with its ST_PENDING cleared. Must
+// be reached by all 64 lanes.
+template<typename TIn, int NMAX, int N_EXACT>
+__device__ void exact_pending_wave(uint64_t pend, int row, int col, int d, const Stack& f0,
+                                   const Stack& f1, Depth<NMAX, N_EXACT> n, float thr, float step,
+                                   int npts, bool mv, float minvar, float& res, uint8_t& st,
+                                   float4* __restrict__ xs) {
+    const int lane = __lane_id();
+    const int P = min(npts, 64), G = min(64 / P, EXACT_GROUPS);
+    const int g = lane / P, k0 = lane - g * P;
+    const int nf = n.get();
+    float xl = -1.0f;  // grid point k0
+    for (int m = 0; m < k0; ++m) xl += step;
+    while (pend) {
+        // the next (up to) G pending pixels: group gg takes source lane src[gg]
+        int ng = 0, head = -1;
+        int prow[EXACT_GROUPS], pcol[EXACT_GROUPS], pc1[EXACT_GROUPS];
+#pragma unroll
+        for (int gg = 0; gg < EXACT_GROUPS; ++gg) {
+            prow[gg] = pcol[gg] = 0;
+            pc1[gg] = 1;
+            if (gg < G && pend) {
+                const int sl = __builtin_ctzll(pend);
+                pend &= pend - 1;
+                ++ng;
+                if (lane == sl) head = gg * P;
+                prow[gg] = __builtin_amdgcn_readlane(row, sl);
+                pcol[gg] = __builtin_amdgcn_readlane(col, sl);
+                pc1[gg] = pcol[gg] - __builtin_amdgcn_readlane(d, sl);
+            }
+        }
+        // stage (u, a, b, c) of every frame of the ng pixels
+        constexpr int PER_LANE = (EXACT_GROUPS * NMAX + 63) / 64;
+#pragma unroll
+        for (int q = 0; q < PER_LANE; ++q) {
+            const int e = q * 64 + lane;
+            const int gg = e / nf, t = e - gg * nf;
+            if (gg < ng) {
+                int r = prow[0], c = pcol[0], c1 = pc1[0];
+#pragma unroll
+                for (int h = 1; h < EXACT_GROUPS; ++h)
+                    if (gg == h) {
+                        r = prow[h];
+                        c = pcol[h];
+                        c1 = pc1[h];
+                    }
+                const float u = (float)load_sample_cached<TIn>(f0, t, r, c);
+                const float y0 = (float)load_sample_cached<TIn>(f1, t, r, c1 - 1);
+                const float y1 = (float)load_sample_cached<TIn>(f1, t, r, c1);
+                const float y2 = (float)load_sample_cached<TIn>(f1, t, r, c1 + 1);
+                xs[gg * NMAX + t] = make_float4(u, 0.5f * (y0 - 2.0f * y1 + y2), 0.5f * (-y0 + y2), y1);
+            }
+        }
+        // the wave's LDS operations complete in order: a fence keeps the
+        // compiler from moving the reads above the writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool on = g < ng;
+        const float4* px = xs + (on ? g : 0) * NMAX;
+        float best = -1.0f;
+        int best_k = 0x7FFFFFFF;  // none: x stays 0 (agree.hpp:159)
+        float x = xl, var0 = 0.0f;
+        for (int k = k0; __ballot(on && k < npts); k += P) {
+            const float xe = k < npts ? x : 0.0f;  // lanes past the grid: any finite point
+            // frame t's interpolated sample at xe (agree.hpp:164)
+            const auto sample = [&](const float4& v) {
+                return (float)(TIn)(int)__builtin_rintf(v.y * xe * xe + v.z * xe + v.w);
+            };
+            float mean0 = 0.0f, mean1 = 0.0f;
+#pragma unroll 8
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) {
+                    const float4 v = px[t];
+                    mean0 += v.x;
+                    mean1 += sample(v);
+                }
+            mean0 /= (float)nf;
+            mean1 /= (float)nf;
+            float covar = 0.0f, var1 = 0.0f;
+            var0 = 0.0f;
+#pragma unroll 8
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) {
+                    const float4 v = px[t];
+                    const float c0 = v.x - mean0, d1 = sample(v) - mean1;
+                    var0 = __builtin_fmaf(c0, c0, var0);
+                    covar = __builtin_fmaf(c0, d1, covar);
+                    var1 = __builtin_fmaf(d1, d1, var1);
+                }
+            const float nxc = (mv && var1 < minvar) ? -1.0f : covar / psqrt(var0 * var1);
+            if (k < npts && best < nxc) {  // lane-local first strict maximum
+                best = nxc;
+                best_k = k;
+            }
+            for (int m = 0; m < P; ++m) x += step;
+        }
+        // the next round's staging must not overtake this round's reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (mv && var0 < minvar) {  // no grid point is scored (agree.hpp:158)
+            best = -1.0f;
+            best_k = 0x7FFFFFFF;
+        }
+        // segmented reduction towards each group's first lane: larger value,
+        // then smaller k
+        const int seg_end = (g + 1) * P;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float ob = __shfl_down(best, o);
+            const int ok = __shfl_down(best_k, o);
+            if (lane + o < seg_end && (ob > best || (ob == best && ok < best_k))) {
+                best = ob;
+                best_k = ok;
+            }
+        }
+        const float hb = __shfl(best, head < 0 ? lane : head);
+        const int hk = __shfl(best_k, head < 0 ? lane : head);
+        if (head >= 0) {
+            float bx = 0.0f;
+            if (hk != 0x7FFFFFFF) {
+                bx = -1.0f;
+                for (int m = 0; m < hk; ++m) bx += step;
+            }
+            res = hb < thr ? NAN : (float)d - bx;
+            st &= (uint8_t)~ST_PENDING;
+        }
+    }
+}
+
+// act: the lane's pixel is real (row and col are valid either way; inactive
+// lanes compute but write nothing). Must be reached by all 64 lanes.
 template<typename TIn, int NMAX, int N_EXACT>
 __device__ __forceinline__ void agree_subpixel_moments_px(
-    int row, int col, const int16_t* __restrict__ disp, int64_t dstep, const Stack& f0,
-    const Stack& f1, int n_rt, int W, float thr, float step, int has_minvar, float minvar,
-    float* __restrict__ out, int64_t ostep, uint8_t* __restrict__ status) {
+    bool act, int row, int col, const int16_t* __restrict__ disp, int64_t dstep, const Stack& f0,
+    const Stack& f1, int n_rt, int W, float thr, float step, int npts, int has_minvar, float minvar,
+    float* __restrict__ out, int64_t ostep, uint8_t* __restrict__ status, float4* __restrict__ xs) {
     const Depth<NMAX, N_EXACT> n{n_rt};
     const bool mv = has_minvar != 0;
     const int d = disp[(size_t)row * dstep + col];
     const int col1 = col - d;
     const bool valid_d = d != INVALID16 && 0 <= col1 && col1 < W;
     const bool interior = valid_d && col1 != 0 && col1 != W - 1;
-    const int c1 = interior ? col1 : 1;  // in range for the loads of other lanes
+    const int c1 = interior ? col1 : 1;  // a valid column for the loads of the other lanes
     float res = NAN;
     uint8_t st = 0;
 
@@ -84,40 +233,37 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     float s0 = 0, s1 = 0, s2 = 0, su0 = 0, su1 = 0, su2 = 0;
     float s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0, su_f = 0;
     float ymin = 3.0e38f, ymax = -3.0e38f;
+    // branch-free loop (the right samples of lanes without an interior match
+    // are read at a valid column and ignored), so that loads of many frames
+    // are in flight at once
     const float u_ref = (float)load_sample_cached<TIn>(f0, 0, row, col);
-    float y_ref0 = 0.0f, y_ref1 = 0.0f, y_ref2 = 0.0f;
-    if (interior) {
-        y_ref0 = (float)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
-        y_ref1 = (float)load_sample_cached<TIn>(f1, 0, row, c1);
-        y_ref2 = (float)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
-    }
-    // 11 frames per round: 44 loads in flight per lane
+    const float y_ref0 = (float)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
+    const float y_ref1 = (float)load_sample_cached<TIn>(f1, 0, row, c1);
+    const float y_ref2 = (float)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
 #pragma unroll 11
     for (int t = 0; t < NMAX; ++t)
         if (n.has(t)) {
             const float uf = (float)load_sample_cached<TIn>(f0, t, row, col);
+            const float f0y = (float)load_sample_cached<TIn>(f1, t, row, c1 - 1);
+            const float f1y = (float)load_sample_cached<TIn>(f1, t, row, c1);
+            const float f2y = (float)load_sample_cached<TIn>(f1, t, row, c1 + 1);
             const double u = uf;
             su += u;
             suu = __builtin_fma(u, u, suu);
-            if (interior) {
-                const float f0y = (float)load_sample_cached<TIn>(f1, t, row, c1 - 1);
-                const float f1y = (float)load_sample_cached<TIn>(f1, t, row, c1);
-                const float f2y = (float)load_sample_cached<TIn>(f1, t, row, c1 + 1);
-                // reach of the frame's parabola on [-1, 1]: the end values and,
-                // where it is convex (concave) with the vertex inside, the
-                // vertex, which lies above (below) y1 -/+ |y2 - y0| / 4
-                const float r4 = 0.25f * fabsf(f2y - f0y);
-                ymin = fminf(ymin, fminf(fminf(f0y, f2y), f1y - r4));
-                ymax = fmaxf(ymax, fmaxf(fmaxf(f0y, f2y), f1y + r4));
-                const float us = uf - u_ref, y0 = f0y - y_ref0, y1 = f1y - y_ref1, y2 = f2y - y_ref2;
-                su_f += us;
-                s0 += y0; s1 += y1; s2 += y2;
-                su0 = __builtin_fmaf(us, y0, su0); su1 = __builtin_fmaf(us, y1, su1);
-                su2 = __builtin_fmaf(us, y2, su2);
-                s00 = __builtin_fmaf(y0, y0, s00); s01 = __builtin_fmaf(y0, y1, s01);
-                s02 = __builtin_fmaf(y0, y2, s02); s11 = __builtin_fmaf(y1, y1, s11);
-                s12 = __builtin_fmaf(y1, y2, s12); s22 = __builtin_fmaf(y2, y2, s22);
-            }
+            // reach of the frame's parabola on [-1, 1]: the end values and,
+            // where it is convex (concave) with the vertex inside, the
+            // vertex, which lies above (below) y1 -/+ |y2 - y0| / 4
+            const float r4 = 0.25f * fabsf(f2y - f0y);
+            ymin = fminf(ymin, fminf(fminf(f0y, f2y), f1y - r4));
+            ymax = fmaxf(ymax, fmaxf(fmaxf(f0y, f2y), f1y + r4));
+            const float us = uf - u_ref, y0 = f0y - y_ref0, y1 = f1y - y_ref1, y2 = f2y - y_ref2;
+            su_f += us;
+            s0 += y0; s1 += y1; s2 += y2;
+            su0 = __builtin_fmaf(us, y0, su0); su1 = __builtin_fmaf(us, y1, su1);
+            su2 = __builtin_fmaf(us, y2, su2);
+            s00 = __builtin_fmaf(y0, y0, s00); s01 = __builtin_fmaf(y0, y1, s01);
+            s02 = __builtin_fmaf(y0, y2, s02); s11 = __builtin_fmaf(y1, y1, s11);
+            s12 = __builtin_fmaf(y1, y2, s12); s22 = __builtin_fmaf(y2, y2, s22);
         }
     const double inv = 1.0 / (double)n.get();
     const double var0d = suu - su * su * inv;
@@ -230,199 +376,47 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             }
         }
     }
-    out[(size_t)row * ostep + col] = res;
-    status[(size_t)row * W + col] = st;
+    const uint64_t pend = __ballot(act && (st & ST_PENDING));
+    if (pend)
+        exact_pending_wave<TIn>(pend, row, col, d, f0, f1, n, thr, step, npts, mv, minvar, res, st, xs);
+    if (act) {
+        out[(size_t)row * ostep + col] = res;
+        status[(size_t)row * W + col] = st;
+    }
 }
 
 template<typename TIn, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree_subpixel_moments(
     const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1, int n_rt,
-    int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
+    int W, float thr, float step, int npts, int has_minvar, float minvar, float* __restrict__ out,
     int64_t ostep, uint8_t* __restrict__ status) {
+    __shared__ float4 xs[4][EXACT_GROUPS * NMAX];
     const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
-    if (col >= W)
+    if ((col & ~63) >= W)  // whole wave past the row
         return;
-    agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, step,
-                                                  has_minvar, minvar, out, ostep, status);
+    agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(col < W, row, min(col, W - 1), disp, dstep, f0, f1,
+                                                  n_rt, W, thr, step, npts, has_minvar, minvar, out,
+                                                  ostep, status, xs[threadIdx.x >> 6]);
 }
 
-// over listed pixels only (row r = blockIdx.x: row_count[r] columns at list + r * W)
+// over listed pixels only (dense numbering of the lists; each wave walks
+// its 64 entries per round, lanes past the end repeat the last entry)
 template<typename TIn, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree_subpixel_moments_list(
     const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1, int n_rt,
-    int W, float thr, float step, int has_minvar, float minvar, float* __restrict__ out,
-    int64_t ostep, uint8_t* __restrict__ status, const int16_t* __restrict__ list,
-    const int* __restrict__ row_count) {
-    const int row = blockIdx.x, i = blockIdx.y * 256 + threadIdx.x;
-    if (i >= row_count[row])
-        return;
-    agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(row, list[(size_t)row * W + i], disp, dstep, f0, f1,
-                                                  n_rt, W, thr, step, has_minvar, minvar, out, ostep,
-                                                  status);
-}
-
-// The exact subpixel stage (agree.hpp:94-185 arithmetic, bit-identical to
-// k_agree_subpixel) for listed pixels, one pixel per 32 lanes: lane j scores
-// grid points j, j + 32, ... (x_k formed by the reference's running float
-// sum) and the lanes reduce to the reference's first strict maximum. Clears
-// ST_PENDING of the pixels it decides (status nullable).
-template<typename TIn, int NMAX, int N_EXACT>
-__device__ __forceinline__ void agree_subpixel_wide_px(
-    int row, int col, int j, const int16_t* __restrict__ disp, int64_t dstep, const Stack& f0,
-    const Stack& f1, int n_rt, int W, float thr, float step, int npts, int has_minvar,
-    float minvar, float* __restrict__ out, int64_t ostep, float* __restrict__ corrmap,
-    int64_t cstep, uint8_t* __restrict__ status) {
-    const Depth<NMAX, N_EXACT> n{n_rt};
-    const bool mv = has_minvar != 0;
-    const int d = disp[(size_t)row * dstep + col];
-    const int col1 = col - d;
-    float res = NAN, corr = NAN;
-    if (d != INVALID16 && 0 <= col1 && col1 < W) {
-        if (col1 == 0 || col1 == W - 1) {
-            corr = nxcorr_stream<TIn>(f0, row, col, [&](int t) {
-                return (float)load_sample_cached<TIn>(f1, t, row, col1);
-            }, n, mv, minvar);
-            if (!(corr < thr)) res = (float)d;
-        } else {
-            // samples held in registers (one load round), the parabola
-            // coefficients, left mean, centred samples and variance formed
-            // as k_agree_subpixel forms them
-            float p0[NMAX], a[NMAX], b[NMAX], c[NMAX];
-#pragma unroll
-            for (int t = 0; t < NMAX; ++t) {
-                p0[t] = a[t] = b[t] = c[t] = 0.0f;
-                if (n.has(t)) {
-                    p0[t] = (float)load_sample_cached<TIn>(f0, t, row, col);
-                    const float y0 = (float)load_sample_cached<TIn>(f1, t, row, col1 - 1);
-                    const float y1 = (float)load_sample_cached<TIn>(f1, t, row, col1);
-                    const float y2 = (float)load_sample_cached<TIn>(f1, t, row, col1 + 1);
-                    a[t] = 0.5f * (y0 - 2.0f * y1 + y2);
-                    b[t] = 0.5f * (-y0 + y2);
-                    c[t] = y1;
-                }
-            }
-            float mean0 = 0.0f;
-#pragma unroll
-            for (int t = 0; t < NMAX; ++t)
-                if (n.has(t)) mean0 += p0[t];
-            mean0 /= (float)n.get();
-            float var0 = 0.0f;
-#pragma unroll
-            for (int t = 0; t < NMAX; ++t)
-                if (n.has(t)) {
-                    p0[t] -= mean0;  // centred from here on
-                    var0 = __builtin_fmaf(p0[t], p0[t], var0);
-                }
-            float best = -1.0f;
-            int best_k = 0x7FFFFFFF;  // none: x stays 0 (agree.hpp:159)
-            if (!(mv && var0 < minvar)) {
-                // x_j by the reference's running sum, then 32 steps per round
-                float x = -1.0f;
-                for (int m = 0; m < j; ++m) x += step;
-                for (int k = j; k < npts; k += 32) {
-                    float it[NMAX], mean1 = 0.0f;
-#pragma unroll
-                    for (int t = 0; t < NMAX; ++t) {
-                        it[t] = 0.0f;
-                        if (n.has(t)) {
-                            it[t] = (float)(TIn)(int)__builtin_rintf(a[t] * x * x + b[t] * x + c[t]);
-                            mean1 += it[t];
-                        }
-                    }
-                    mean1 /= (float)n.get();
-                    float covar = 0.0f, var1 = 0.0f;
-#pragma unroll
-                    for (int t = 0; t < NMAX; ++t)
-                        if (n.has(t)) {
-                            const float d1 = it[t] - mean1;
-                            covar = __builtin_fmaf(p0[t], d1, covar);
-                            var1 = __builtin_fmaf(d1, d1, var1);
-                        }
-                    const float nxc = (mv && var1 < minvar) ? -1.0f : covar / psqrt(var0 * var1);
-                    if (best < nxc) {  // lane-local first strict maximum
-                        best = nxc;
-                        best_k = k;
-                    }
-                    for (int m = 0; m < 32; ++m) x += step;
-                }
-            }
-            // reduce over the pixel's 32 lanes: larger value, then smaller k
-#pragma unroll
-            for (int o = 16; o > 0; o >>= 1) {
-                const float ob = __shfl_xor(best, o, 32);
-                const int ok = __shfl_xor(best_k, o, 32);
-                if (ob > best || (ob == best && ok < best_k)) {
-                    best = ob;
-                    best_k = ok;
-                }
-            }
-            float bx = 0.0f;
-            if (best_k != 0x7FFFFFFF) {
-                bx = -1.0f;
-                for (int k = 0; k < best_k; ++k) bx += step;
-            }
-            corr = best;
-            if (!(best < thr)) res = (float)d - bx;
-        }
-    }
-    if (j == 0) {
-        out[(size_t)row * ostep + col] = res;
-        if (corrmap) corrmap[(size_t)row * cstep + col] = corr;
-        if (status) status[(size_t)row * W + col] &= (uint8_t)~ST_PENDING;
-    }
-}
-
-
-template<typename TIn, int NMAX, int N_EXACT>
-__global__ __launch_bounds__(256) void k_agree_subpixel_wide(
-    const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1, int n_rt,
     int W, float thr, float step, int npts, int has_minvar, float minvar, float* __restrict__ out,
-    int64_t ostep, float* __restrict__ corrmap, int64_t cstep, const int16_t* __restrict__ list,
-    const int* __restrict__ row_count, uint8_t* __restrict__ status) {
-    const int row = blockIdx.x, j = threadIdx.x & 31;
-    const int count = row_count[row];
-    for (int i = blockIdx.y * 8 + (threadIdx.x >> 5); i < count; i += gridDim.y * 8)  // uniform per pixel
-        agree_subpixel_wide_px<TIn, NMAX, N_EXACT>(row, list[(size_t)row * W + i], j, disp, dstep, f0, f1,
-                                                   n_rt, W, thr, step, npts, has_minvar, minvar, out,
-                                                   ostep, corrmap, cstep, status);
-}
-
-// Per-row lists (row_count[r] columns at list + r * W, any order; row_count
-// zeroed before) of the pixels with a status bit in `pending_mask`, or
-// (pending_mask == 0) of the fallback set: invalid result and not hopeless.
-// The row is blockIdx.x so that rows spread over the XCDs.
-__global__ __launch_bounds__(256) void k_status_list(const uint8_t* __restrict__ status,
-                                                     const float* __restrict__ res, int64_t rstep,
-                                                     int pending_mask, int W,
-                                                     int16_t* __restrict__ list,
-                                                     int* __restrict__ row_count) {
-    __shared__ int wsum[4], base;
-    const int row = blockIdx.x, col = blockIdx.y * 256 + threadIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    bool take = false;
-    if (col < W) {
-        const uint8_t st = status[(size_t)row * W + col];
-        take = pending_mask ? (st & pending_mask) != 0
-                            : (__builtin_isnan(res[(size_t)row * rstep + col]) && !(st & ST_HOPELESS));
+    int64_t ostep, uint8_t* __restrict__ status, const RowLists l) {
+    __shared__ float4 xs[4][EXACT_GROUPS * NMAX];
+    const int total = l.row_start[l.H];
+    const int lane = threadIdx.x & 63;
+    for (int wbase = blockIdx.x * 256 + (threadIdx.x & ~63); wbase < total; wbase += gridDim.x * 256) {
+        const int i = wbase + lane;
+        int row, col;
+        list_entry(l, min(i, total - 1), row, col);
+        agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(i < total, row, col, disp, dstep, f0, f1, n_rt, W,
+                                                      thr, step, npts, has_minvar, minvar, out, ostep,
+                                                      status, xs[threadIdx.x >> 6]);
     }
-    const uint64_t m = __ballot(take);
-    if (lane == 0) wsum[wave] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        base = total ? atomicAdd(&row_count[row], total) : 0;
-    }
-    __syncthreads();
-    if (take) {
-        int off = __popcll(m & ((1ull << lane) - 1));
-        for (int w = 0; w < wave; ++w) off += wsum[w];
-        list[(size_t)row * W + base + off] = (int16_t)col;
-    }
-}
-
-__global__ void k_zero_i32(int* p, int count) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < count) p[i] = 0;
 }
 
 }  // namespace bicos_hip

@@ -398,7 +398,7 @@ int run_subpixel_moments_t(const int16_t* disp, int64_t dstep, const Stack& f0, 
     return with_nmax(n, [&](auto NM, auto NE) -> int {
         constexpr int NMAX = decltype(NM)::value, NEX = decltype(NE)::value;
         LAUNCH("agree_subpixel_moments", (k_agree_subpixel_moments<TIn, NMAX, NEX>), dim3((W + 255) / 256, H), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, status);
+               disp, dstep, f0, f1, (int)n, W, thr, step, subpixel_points(step), has_mv, mv, out, ostep, status);
         return BICOS_OK;
     });
 }
@@ -409,21 +409,8 @@ int run_moments_list_t(const int16_t* disp, int64_t dstep, const Stack& f0, cons
                        float* out, int64_t ostep, uint8_t* status, const int16_t* list,
                        const int* row_count, hipStream_t s) {
     return with_nmax(n, [&](auto NM, auto NE) -> int {
-        LAUNCH("agree_subpixel_moments_list", (k_agree_subpixel_moments_list<TIn, decltype(NM)::value, decltype(NE)::value>), dim3(H, (W + 255) / 256), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, status, list, row_count);
-        return BICOS_OK;
-    });
-}
-
-template<typename TIn>
-int run_wide_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, uint32_t n,
-               int W, int H, float thr, float step, int has_mv, float mv, float* out, int64_t ostep,
-               float* corr, int64_t cstep, const int16_t* list, const int* row_count,
-               uint8_t* status, hipStream_t s) {
-    const int npts = subpixel_points(step);
-    return with_nmax(n, [&](auto NM, auto NE) -> int {
-        LAUNCH("agree_subpixel_wide", (k_agree_subpixel_wide<TIn, decltype(NM)::value, decltype(NE)::value>), dim3(H, 8), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, step, npts, has_mv, mv, out, ostep, corr, cstep, list, row_count, status);
+        LAUNCH("agree_subpixel_moments_list", (k_agree_subpixel_moments_list<TIn, decltype(NM)::value, decltype(NE)::value>), dim3(LIST_BLOCKS), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, subpixel_points(step), has_mv, mv, out, ostep, status, row_lists(list, row_count, W, H));
         return BICOS_OK;
     });
 }
@@ -437,36 +424,43 @@ int run_moments_only(const int16_t* disp, int64_t dstep, const Stack& f0, const 
         : run_subpixel_moments_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s);
 }
 
-// the pending pixels of the MOMENTS status, through the exact stage
-int run_pending_exact(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
-                      int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
-                      float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
-                      hipStream_t s) {
-    int rc;
-    if ((rc = run_status_list(ms->status, nullptr, 0, ST_PENDING, W, H, ms->list, ms->row_count, s)))
-        return rc;
-    return depth == BICOS_8U
-        ? run_wide_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, nullptr, 0, ms->list, ms->row_count, ms->status, s)
-        : run_wide_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, nullptr, 0, ms->list, ms->row_count, ms->status, s);
-}
-
 int run_moments_listed(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                        int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
                        float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
                        const int16_t* list, const int* row_count, hipStream_t s) {
-    int rc = depth == BICOS_8U
+    return depth == BICOS_8U
         ? run_moments_list_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, list, row_count, s)
         : run_moments_list_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, list, row_count, s);
-    if (rc) return rc;
-    return run_pending_exact(disp, dstep, f0, f1, depth, n, W, H, thr, step, has_mv, mv_n, out,
-                             ostep, ms, s);
 }
 
-int run_status_list(const uint8_t* status, const float* res, int64_t rstep, int mask, int W, int H,
-                    int16_t* list, int* row_count, hipStream_t s) {
-    LAUNCH("status_list", k_zero_i32, dim3((H + 255) / 256), dim3(256), 0, s, row_count, H);
-    LAUNCH("status_list", k_status_list, dim3(H, (W + 255) / 256), dim3(256), 0, s, status, res,
-           rstep, mask, W, list, row_count);
+// row_start = exclusive prefix sum of row_count (one workgroup: each thread
+// sums a contiguous chunk of rows, the chunk totals are scanned in LDS)
+__global__ __launch_bounds__(1024) void k_row_scan(int* __restrict__ counts, int H) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (H + 1023) / 1024;
+    const int r0 = min(t * chunk, H), r1 = min(r0 + chunk, H);
+    int sum = 0;
+    for (int r = r0; r < r1; ++r) sum += counts[r];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int* start = counts + H;
+    int run = part[t] - sum;
+    for (int r = r0; r < r1; ++r) {
+        start[r] = run;
+        run += counts[r];
+    }
+    if (t == 1023) start[H] = part[1023];
+}
+
+int run_row_scan(int* row_count, int H, hipStream_t s) {
+    LAUNCH("row_scan", k_row_scan, dim3(1), dim3(1024), 0, s, row_count, H);
     return BICOS_OK;
 }
 
@@ -475,14 +469,10 @@ int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stac
                  float mv_n, int precision, float* out, int64_t ostep, void* corr, int64_t cstep,
                  hipStream_t s, const MomentsScratch* ms) {
     if (precision == BICOS_PRECISION_MOMENTS && !corr) {
-        // closed-form decisions, then the exact stage on the pending pixels
+        // closed-form decisions, the exact stage on the pending pixels
         if (!ms) return fail(BICOS_ERR_INVALID_ARGUMENT, "MOMENTS needs its scratch buffers");
-        int rc = depth == BICOS_8U
-            ? run_subpixel_moments_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s)
-            : run_subpixel_moments_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, ms->status, s);
-        if (rc) return rc;
-        return run_pending_exact(disp, dstep, f0, f1, depth, n, W, H, thr, step, has_mv, mv_n,
-                                 out, ostep, ms, s);
+        return run_moments_only(disp, dstep, f0, f1, depth, n, W, H, thr, step, has_mv, mv_n, out,
+                                ostep, ms, s);
     }
     if (precision == BICOS_PRECISION_MOMENTS)  // a corrmap is written: the exact stage throughout
         precision = BICOS_PRECISION_SINGLE;
@@ -506,8 +496,8 @@ int run_agree_list_t(int16_t* disp, int64_t dstep, const Stack& f0, const Stack&
                      int W, float thr, int has_mv, float mv, TPrec* corr, int64_t cstep,
                      const int16_t* list, const int* row_count, int H, hipStream_t s) {
     return with_nmax(n, [&](auto NM, auto NE) -> int {
-        LAUNCH("agree_list", (k_agree_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(H, (W + 255) / 256), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, has_mv, mv, corr, cstep, list, row_count);
+        LAUNCH("agree_list", (k_agree_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(LIST_BLOCKS), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, has_mv, mv, corr, cstep, row_lists(list, row_count, W, H));
         return BICOS_OK;
     });
 }
@@ -518,8 +508,8 @@ int run_subpixel_list_t(const int16_t* disp, int64_t dstep, const Stack& f0, con
                         int64_t ostep, TPrec* corr, int64_t cstep, const int16_t* list,
                         const int* row_count, int H, hipStream_t s) {
     return with_nmax(n, [&](auto NM, auto NE) -> int {
-        LAUNCH("agree_subpixel_list", (k_agree_subpixel_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(H, (W + 255) / 256), dim3(256), 0, s,
-               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep, list, row_count);
+        LAUNCH("agree_subpixel_list", (k_agree_subpixel_list<TIn, TPrec, decltype(NM)::value, decltype(NE)::value>), dim3(LIST_BLOCKS), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, has_mv, mv, out, ostep, corr, cstep, row_lists(list, row_count, W, H));
         return BICOS_OK;
     });
 }
@@ -705,7 +695,7 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     const size_t rev_bytes = consistency ? align_up(npx * 2, 256) : 0;
     const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
     const bool moments = subpix && cfg->precision == BICOS_PRECISION_MOMENTS;
-    const size_t mom_bytes = moments ? MomentsScratch::bytes(npx, H) : 0;
+    const size_t mom_bytes = moments ? MomentsScratch::bytes(npx) : 0;
 
     int dev;
     if ((rc = current_device(&dev))) return rc;
@@ -719,7 +709,7 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     int16_t* rev = consistency ? (int16_t*)(p + 2 * desc_bytes + raw_bytes) : nullptr;
     uint8_t* pack0 = p + 2 * desc_bytes + raw_bytes + rev_bytes;
     uint8_t* pack1 = pack0 + pk0;
-    const MomentsScratch ms = MomentsScratch::carve(pack1 + pk1, npx);
+    const MomentsScratch ms = MomentsScratch::carve(pack1 + pk1);
 
     auto body = [&]() -> int {
         int r;
@@ -860,11 +850,11 @@ int bicos_hip_agree_subpixel(const int16_t* disparity, const bicos_image* stack0
     hipStream_t s = (hipStream_t)stream;
     const size_t npx = (size_t)i[0].rows * i[0].cols;
     return with_stacks(f, i, 2, n, s, [&](const Stack* v, void* extra) {
-        const MomentsScratch ms = MomentsScratch::carve(extra, npx);
+        const MomentsScratch ms = MomentsScratch::carve(extra);
         return run_subpixel(disparity, i[0].cols, v[0], v[1], i[0].depth, n, i[0].cols, i[0].rows,
                             thr, subpixel_step, has_min_variance, mv_n, precision, out,
                             i[0].cols, corrmap, i[0].cols, s, &ms);
-    }, precision == BICOS_PRECISION_MOMENTS ? MomentsScratch::bytes(npx, i[0].rows) : 0);
+    }, precision == BICOS_PRECISION_MOMENTS ? MomentsScratch::bytes(npx) : 0);
 }
 
 int bicos_hip_timing_enable(int32_t enable) {

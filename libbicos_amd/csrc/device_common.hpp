@@ -53,6 +53,27 @@ __global__ __launch_bounds__(256) void k_pack_frames(const FrameTable f, int W, 
         if (col + k < W) dst[col + k] = src[col + k];
 }
 
+// Per-row pixel lists: row r holds row_count[r] columns at cols + r * W (any
+// order); row_start, when built (k_row_scan), is the exclusive prefix sum of
+// row_count (rows + 1 values), which numbers all entries densely.
+struct RowLists {
+    const int16_t* cols;
+    const int* row_count;
+    const int* row_start;
+    int W, H;
+};
+
+// entry i (< row_start[H]) of the dense numbering: its row and column
+__device__ __forceinline__ void list_entry(const RowLists& l, int i, int& row, int& col) {
+    int lo = 0, hi = l.H - 1;  // largest r with row_start[r] <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (l.row_start[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    row = lo;
+    col = l.cols[(size_t)lo * l.W + (i - l.row_start[lo])];
+}
+
 __device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
     // lowered to v_med3_u32
     return max(min(a, b), min(max(a, b), c));

@@ -108,40 +108,36 @@ int search_flags(const uint32_t* d0, const uint32_t* d1, int H, int W, int words
 int run_agree(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1, int depth,
               uint32_t n, int W, int H, float thr, int has_mv, float mv_n, int precision,
               void* corr, int64_t cstep, hipStream_t s);
-// Scratch of the MOMENTS subpixel stage: per-pixel status (ST_PENDING,
-// ST_HOPELESS) and the per-row list of pending pixels.
+// Per-row pixel lists keep their counts in one int buffer of 2 * rows + 1
+// values: row_count[r] (rows values), then row_start, the exclusive prefix
+// sum (rows + 1 values) that numbers every entry densely; the list kernels
+// walk that numbering with a fixed grid, so one crowded row does not set
+// their duration.
+inline size_t list_counts_bytes(int rows) { return align_up((size_t)(2 * rows + 1) * 4, 256); }
+inline RowLists row_lists(const int16_t* list, const int* row_count, int W, int H) {
+    return RowLists{list, row_count, row_count + H, W, H};
+}
+// row_count + H = prefix sums of row_count[0 .. H)
+int run_row_scan(int* row_count, int H, hipStream_t s);
+// workgroups of the list kernels (grid-stride over the dense numbering)
+constexpr int LIST_BLOCKS = 2048;
+
+// Scratch of the MOMENTS subpixel stage: per-pixel status (ST_HOPELESS:
+// the left variance rejects every match; read by the fallback rule)
 struct MomentsScratch {
-    uint8_t* status;    // rows * cols
-    int16_t* list;      // rows * cols
-    int* row_count;     // rows
-    static size_t bytes(size_t npx, int rows) {
-        return align_up(npx, 256) + align_up(npx * 2, 256) + align_up((size_t)rows * 4, 256);
-    }
-    static MomentsScratch carve(void* p, size_t npx) {
-        uint8_t* b = (uint8_t*)p;
-        MomentsScratch m;
-        m.status = b;
-        m.list = (int16_t*)(b + align_up(npx, 256));
-        m.row_count = (int*)(b + align_up(npx, 256) + align_up(npx * 2, 256));
-        return m;
-    }
+    uint8_t* status;  // rows * cols
+    static size_t bytes(size_t npx) { return align_up(npx, 256); }
+    static MomentsScratch carve(void* p) { return MomentsScratch{(uint8_t*)p}; }
 };
 
-// Per-row lists (row_count[r] columns at list + r * W) of the pixels whose
-// MOMENTS status has a bit of `mask`, or (mask 0) of the fallback set:
-// res NaN and not hopeless.
-int run_status_list(const uint8_t* status, const float* res, int64_t rstep, int mask, int W, int H,
-                    int16_t* list, int* row_count, hipStream_t s);
-
-// MOMENTS closed-form stage alone (status written; pending pixels left NaN)
+// MOMENTS subpixel stage of every pixel (closed form, exact stage on the
+// pixels it leaves pending), writing the status
 int run_moments_only(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                      int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
                      float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,
                      hipStream_t s);
 
-// MOMENTS stage of listed pixels (row r: row_count[r] columns at
-// list + r * W), writing their status, then the exact stage on those left
-// pending
+// the same for listed pixels only (lists with row_start built)
 int run_moments_listed(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                        int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
                        float mv_n, float* out, int64_t ostep, const MomentsScratch* ms,

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <random>
 #include <string>
 
@@ -33,8 +34,9 @@ int check_options(const bicos_window_options* o, int depth, float* in_scale, int
         return fail(BICOS_ERR_INVALID_ARGUMENT, "window bins must be in [9, 1024]");
     if (o->hw_lo < 0 || o->hw_hi < 0 || o->hw_lo > 32767 || o->hw_hi > 32767)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "window half-widths must be in [0, 32767]");
-    if (o->fallback != 0 && o->fallback != 1)
-        return fail(BICOS_ERR_INVALID_ARGUMENT, "fallback must be 0 or 1");
+    if (o->fallback != BICOS_FALLBACK_NONE && o->fallback != BICOS_FALLBACK_ALL &&
+        o->fallback != BICOS_FALLBACK_UNSURE)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown fallback mode");
     if (!(o->in_scale >= 0.0f) || !std::isfinite(o->in_scale))
         return fail(BICOS_ERR_INVALID_ARGUMENT, "bad input scale");
     *in_scale = o->in_scale > 0.0f ? o->in_scale : 8.0f / (depth == BICOS_8U ? 255.0f : 65535.0f);
@@ -213,6 +215,28 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
     return BICOS_OK;
 }
 
+/* --------------------------------------------------------- side stream */
+
+// Per device: a second stream (default priority) on which the network runs
+// while the descriptor transform runs on the caller's stream, and the fork /
+// join events. Guarded by g_ws_mu (held for the whole call).
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+std::map<int, SideStream> g_side;
+
+int side_stream(int dev, SideStream** out) {
+    SideStream& ss = g_side[dev];
+    if (!ss.stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&ss.stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+    }
+    *out = &ss;
+    return BICOS_OK;
+}
+
 /* ------------------------------------------------------- windowed search */
 
 template<int WORDS>
@@ -303,26 +327,27 @@ __global__ void k_zero_u32(unsigned int* p, int count) {
     if (i < count) p[i] = 0u;
 }
 
-// status non-null (MOMENTS): the set from the NCC stage's status bits
+// status non-null (MOMENTS): the set from the NCC stage's status bits;
+// hw non-null: only pixels of cells given the wide window
 int run_fallback_list(const float* disp_f, const int16_t* disp_i, int64_t dstep, const Stack& v0,
                       int depth, uint32_t n, const Grid& g, int has_mv, float mv_n, float thr,
-                      int precision, const uint8_t* status, int16_t* list, int* row_count,
-                      hipStream_t s) {
+                      int precision, const uint8_t* status, const int16_t* hw, int hw_lo,
+                      int16_t* list, int* row_count, hipStream_t s) {
     LAUNCH("fallback_list", k_zero_u32, dim3((g.H + 255) / 256), dim3(256), 0, s, (unsigned int*)row_count, g.H);
     const dim3 grid(g.H, (g.W + 255) / 256), block(256);
     if (status) {
         LAUNCH("fallback_list", k_fallback_list_status, grid, block, 0, s, status, disp_f, dstep, g.W,
-               list, row_count);
-        return BICOS_OK;
+               hw, hw_lo, g.Hq, g.Wq, list, row_count);
+        return run_row_scan(row_count, g.H, s);
     }
-#define FL(TI, TP) LAUNCH("fallback_list", (k_fallback_list<TI, TP>), grid, block, 0, s, disp_f, disp_i, dstep, v0, (int)n, g.W, has_mv, mv_n, thr, list, row_count)
+#define FL(TI, TP) LAUNCH("fallback_list", (k_fallback_list<TI, TP>), grid, block, 0, s, disp_f, disp_i, dstep, v0, (int)n, g.W, has_mv, mv_n, thr, hw, hw_lo, g.Hq, g.Wq, list, row_count)
     if (precision == BICOS_PRECISION_DOUBLE) {
         if (depth == BICOS_8U) FL(uint8_t, double); else FL(uint16_t, double);
     } else {
         if (depth == BICOS_8U) FL(uint8_t, float); else FL(uint16_t, float);
     }
 #undef FL
-    return BICOS_OK;
+    return run_row_scan(row_count, g.H, s);
 }
 
 }  // namespace
@@ -348,7 +373,7 @@ void bicos_hip_window_options_default(bicos_window_options* o) {
     o->hw_lo = 6;
     o->hw_hi = 12;
     o->bins = CNN_K;
-    o->fallback = 1;
+    o->fallback = BICOS_FALLBACK_ALL;
 }
 
 int bicos_hip_window_init(float* p, uint64_t seed) {
@@ -531,10 +556,10 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
     const size_t raw_bytes = subpix ? align_up(npx * 2, 256) : 0;
     const size_t cell_bytes = align_up(g.cells * 4, 256) + align_up(g.cells * 2, 256);
     const bool moments = subpix && cfg->precision == BICOS_PRECISION_MOMENTS;
-    const size_t fb_bytes = fallback ? align_up(npx * 2, 256) + align_up((size_t)H * 4, 256) + 256 +
+    const size_t fb_bytes = fallback ? align_up(npx * 2, 256) + list_counts_bytes(H) + 256 +
                                            (consistency ? 2 * align_up(npx * 2, 256) : 0)
                                      : 0;
-    const size_t mom_bytes = moments ? MomentsScratch::bytes(npx, H) : 0;
+    const size_t mom_bytes = moments ? MomentsScratch::bytes(npx) : 0;
     const size_t cnn_bytes = align_up(windows_scratch_bytes(g), 256);
     const size_t pk0 = align_up(packed_bytes(f0, n, i0), 256), pk1 = align_up(packed_bytes(f1, n, i1), 256);
 
@@ -555,14 +580,14 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
     unsigned long long* fb_total = nullptr;
     if (fallback) {
         list = (int16_t*)p; p += align_up(npx * 2, 256);
-        row_count = (int*)p; p += align_up((size_t)H * 4, 256);
+        row_count = (int*)p; p += list_counts_bytes(H);
         fb_total = (unsigned long long*)p; p += 256;
         if (consistency) {
             fwd = (int16_t*)p; p += align_up(npx * 2, 256);
             rev = (int16_t*)p; p += align_up(npx * 2, 256);
         }
     }
-    const MomentsScratch ms = MomentsScratch::carve(p, npx);
+    const MomentsScratch ms = MomentsScratch::carve(p);
     p += mom_bytes;
     void* cnn_scratch = p; p += cnn_bytes;
     uint8_t* pack0 = p;
@@ -573,10 +598,20 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         Stack v0, v1;
         if ((r = make_stack(f0, n, i0, pack0, &v0, s))) return r;
         if ((r = make_stack(f1, n, i1, pack1, &v1, s))) return r;
-        if ((r = transform_any(v0, &v1, i0.depth, n, cfg->mode, W, H, words, desc0, desc1, s))) return r;
-        if ((r = cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch, center, hw,
-                             nullptr, s)))
-            return r;
+        // the network (reads the frames only) on the side stream while the
+        // transform runs on the caller's
+        SideStream* ss;
+        if ((r = side_stream(dev, &ss))) return r;
+        HIP_TRY(hipEventRecord(ss->fork, s));
+        HIP_TRY(hipStreamWaitEvent(ss->stream, ss->fork, 0));
+        const int rc_cnn = cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch,
+                                       center, hw, nullptr, ss->stream);
+        // joined in every case: the workspace is released on the caller's stream
+        HIP_TRY(hipEventRecord(ss->join, ss->stream));
+        r = rc_cnn ? rc_cnn
+                   : transform_any(v0, &v1, i0.depth, n, cfg->mode, W, H, words, desc0, desc1, s);
+        HIP_TRY(hipStreamWaitEvent(s, ss->join, 0));
+        if (r) return r;
         int16_t* d16 = subpix ? raw : (int16_t*)disparity->data;
         const int64_t d16_step = subpix ? W : disparity->step / 2;
         if ((r = run_window_search(desc0, desc1, g, words, nodupes, consistency, max_lr, center, hw,
@@ -589,12 +624,7 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         const int64_t cstep = cm ? corrmap->step / elem_size(corrmap->type) : 0;
         float* outf = subpix ? (float*)disparity->data : nullptr;
         const int64_t ostep = subpix ? disparity->step / 4 : 0;
-        if (moments && !cm && fallback)
-            // the pixels it leaves pending join the fallback set (whose
-            // results then equal the plain path's) and are decided there
-            r = run_moments_only(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
-                                 cfg->subpixel_step, cfg->has_min_variance, mv_n, outf, ostep, &ms, s);
-        else if (subpix)
+        if (subpix)
             r = run_subpixel(raw, W, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
                              cfg->subpixel_step, cfg->has_min_variance, mv_n, cfg->precision, outf,
                              ostep, cm, cstep, s, moments ? &ms : nullptr);
@@ -604,10 +634,12 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         if (r || !fallback) return r;
 
         // fallback: full-row search and NCC stage of the rejected pixels
+        const bool unsure = opts->fallback == BICOS_FALLBACK_UNSURE;
         if ((r = run_fallback_list(outf, subpix ? nullptr : d16, subpix ? ostep : d16_step, v0,
                                    i0.depth, n, g, cfg->has_min_variance, mv_n,
                                    cfg->nxcorr_threshold, cfg->precision,
-                                   moments && !cm ? ms.status : nullptr, list, row_count, s)))
+                                   moments && !cm ? ms.status : nullptr, unsure ? hw : nullptr,
+                                   opts->hw_lo, list, row_count, s)))
             return r;
         if (consistency) {
             if ((r = run_search_list(desc0, desc1, H, W, words, nodupes, SEARCH_REV, list, row_count,
@@ -616,7 +648,7 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
             if ((r = run_search_list(desc1, desc0, H, W, words, nodupes, SEARCH_REV, fwd, row_count,
                                      rev, nullptr, 0, s)))
                 return r;
-            LAUNCH("lr_combine", k_lr_combine, dim3(H, (W + 255) / 256), dim3(256), 0, s,
+            LAUNCH("lr_combine", k_lr_combine, dim3(H, 1), dim3(256), 0, s,
                    (const int16_t*)list, (const int*)row_count, W, (const int16_t*)fwd,
                    (const int16_t*)rev, max_lr, d16, d16_step);
         } else {

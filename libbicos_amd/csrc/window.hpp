@@ -11,6 +11,7 @@
 #pragma once
 
 #include "agree.hpp"
+#include "agree_moments.hpp"
 #include "cnn.hpp"
 #include "device_common.hpp"
 #include "search.hpp"
@@ -92,9 +93,17 @@ __global__ __launch_bounds__(256) void k_window_search(const uint32_t* __restric
     out[(size_t)row * ostep + col] = v;
 }
 
+// Whether pixel (row, col) lies in a cell the network was unsure about: it
+// got the wide window (half-width other than hw_lo)
+__device__ __forceinline__ bool cell_unsure(const int16_t* hw_q, int hw_lo, int Hq, int Wq, int row,
+                                            int col) {
+    return hw_q[min(row / CNN_DS, Hq - 1) * Wq + min(col / CNN_DS, Wq - 1)] != hw_lo;
+}
+
 // Fallback set: pixels whose windowed result is invalid and whose left
 // variance does not already reject every match (has_minvar and
-// var0 < minvar, computed as the NCC stage does in precision TPrec). Each
+// var0 < minvar, computed as the NCC stage does in precision TPrec), and,
+// with hw_q given, that lie in a cell the network was unsure about. Each
 // block takes 256 columns of one row (the row is blockIdx.x so that rows
 // spread over the XCDs) and appends its pixels to the row's list
 // (row_count[row] entries at list + row * W, in no particular order);
@@ -104,7 +113,8 @@ __global__ __launch_bounds__(256) void k_fallback_list(const float* __restrict__
                                                        const int16_t* __restrict__ disp_i,
                                                        int64_t dstep, const Stack f0, int n, int W,
                                                        int has_minvar, float minvar, float thr,
-                                                       int16_t* __restrict__ list,
+                                                       const int16_t* __restrict__ hw_q, int hw_lo,
+                                                       int Hq, int Wq, int16_t* __restrict__ list,
                                                        int* __restrict__ row_count) {
     __shared__ int wsum[4], base;
     const int row = blockIdx.x, col = blockIdx.y * 256 + threadIdx.x;
@@ -113,6 +123,7 @@ __global__ __launch_bounds__(256) void k_fallback_list(const float* __restrict__
     if (col < W) {
         const size_t i = (size_t)row * dstep + col;
         need = disp_f ? __builtin_isnan(disp_f[i]) : disp_i[i] == INVALID16;
+        if (need && hw_q) need = cell_unsure(hw_q, hw_lo, Hq, Wq, row, col);
         if (need && has_minvar && thr > -1.0f) {
             // var0 as the NCC stage forms it: mean, then an fma chain of the
             // centred samples in frame order
@@ -150,17 +161,21 @@ __global__ __launch_bounds__(256) void k_fallback_list(const float* __restrict__
     }
 }
 
-// Fallback set from the MOMENTS status: invalid result and not hopeless
-// (ST_HOPELESS = 2). As k_fallback_list without touching the stacks.
+// Fallback set from the MOMENTS status: invalid result and not hopeless,
+// and, with hw_q given, in a cell the network was unsure about. As
+// k_fallback_list without touching the stacks.
 __global__ __launch_bounds__(256) void k_fallback_list_status(
     const uint8_t* __restrict__ status, const float* __restrict__ res, int64_t rstep, int W,
-    int16_t* __restrict__ list, int* __restrict__ row_count) {
+    const int16_t* __restrict__ hw_q, int hw_lo, int Hq, int Wq, int16_t* __restrict__ list,
+    int* __restrict__ row_count) {
     __shared__ int wsum[4], base;
     const int row = blockIdx.x, col = blockIdx.y * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     bool take = false;
     if (col < W) {
-        take = __builtin_isnan(res[(size_t)row * rstep + col]) && !(status[(size_t)row * W + col] & 2);
+        take = __builtin_isnan(res[(size_t)row * rstep + col]) && !(status[(size_t)row * W + col] & ST_HOPELESS);
+        if (take && hw_q)
+            take = cell_unsure(hw_q, hw_lo, Hq, Wq, row, col);
     }
     const uint64_t m = __ballot(take);
     if (lane == 0) wsum[wave] = __popcll(m);
@@ -248,15 +263,15 @@ __global__ __launch_bounds__(256) void k_lr_combine(const int16_t* __restrict__ 
                                                     const int16_t* __restrict__ fwd,
                                                     const int16_t* __restrict__ rev, int max_lr_diff,
                                                     int16_t* __restrict__ disp, int64_t dstep) {
-    const int row = blockIdx.x, i = blockIdx.y * 256 + threadIdx.x;  // rows spread over XCDs
-    if (i >= row_count[row])
-        return;
-    const size_t e = (size_t)row * W + i;
-    const int col = list[e], best = fwd[e], r0 = rev[e];
-    int16_t v = INVALID16;
-    if (best != INVALID16 && r0 != INVALID16 && abs(col - r0) <= max_lr_diff)
-        v = (int16_t)((col + r0) / 2 - best);
-    disp[(size_t)row * dstep + col] = v;
+    const int row = blockIdx.x, count = row_count[row];  // rows spread over XCDs
+    for (int i = blockIdx.y * 256 + threadIdx.x; i < count; i += gridDim.y * 256) {
+        const size_t e = (size_t)row * W + i;
+        const int col = list[e], best = fwd[e], r0 = rev[e];
+        int16_t v = INVALID16;
+        if (best != INVALID16 && r0 != INVALID16 && abs(col - r0) <= max_lr_diff)
+            v = (int16_t)((col + r0) / 2 - best);
+        disp[(size_t)row * dstep + col] = v;
+    }
 }
 
 // cell labels for training: mean of the valid disparities of the 4 x 4

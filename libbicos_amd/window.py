@@ -41,11 +41,17 @@ class WindowOptions:
     hw_lo: int = 6             # half-width (px) of confident cells
     hw_hi: int = 12            # ... and of the others
     bins: int = 80             # correlation offsets: disparities 4 * (-8 .. bins - 9) px
-    fallback: bool = True      # full-row search of the rejected pixels
+    fallback: int = 1          # FALLBACK_NONE / FALLBACK_ALL / FALLBACK_UNSURE (bools: ALL / NONE)
 
     def _c(self) -> _WindowOptions:
+        fb = self.fallback
+        fb = FALLBACK_ALL if fb is True else (FALLBACK_NONE if fb is False else int(fb))
         return _WindowOptions(float(self.in_scale), float(self.p_conf), int(self.hw_lo),
-                              int(self.hw_hi), int(self.bins), 1 if self.fallback else 0)
+                              int(self.hw_hi), int(self.bins), fb)
+
+
+# which rejected pixels get the full-row search (include/bicos_hip.h)
+FALLBACK_NONE, FALLBACK_ALL, FALLBACK_UNSURE = 0, 1, 2
 
 
 def _ptr(t) -> C.c_void_p:
@@ -366,6 +372,8 @@ def _main(argv=None):
     e.add_argument("--hw-lo", type=int, default=None)
     e.add_argument("--hw-hi", type=int, default=None)
     e.add_argument("--p-conf", type=float, default=None)
+    e.add_argument("--fallback", type=int, default=FALLBACK_ALL,
+                   help="0 none, 1 all rejected pixels, 2 those in unsure cells")
     e.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
@@ -384,6 +392,7 @@ def _main(argv=None):
             o.hw_hi = a.hw_hi
         if a.p_conf is not None:
             o.p_conf = a.p_conf
+        o.fallback = a.fallback
         res = evaluate(params, scenes=a.scenes, opts=o, precision=a.precision,
                        log=lambda m: print(m, flush=True))
         print(json.dumps(res), flush=True)
