@@ -69,6 +69,58 @@ __global__ __launch_bounds__(256) void k_cnn_input(const Stack f, int W, int Hq,
     for (int t = 0; t < CNN_IN; ++t) x0[t * plane + (size_t)Y * Wq + X] = bf16_round((q[t] - m) * in_scale);
 }
 
+// k_cnn_input for both views at once (view = blockIdx.z), written
+// cell-major as BF16 bits [Y][X][CNN_IN]; VEC: the 4 samples of a cell row
+// come in one load (stack base, frame stride and row step all multiples of
+// 4 samples). Same arithmetic as k_cnn_input.
+template<typename TIn, bool VEC>
+__global__ __launch_bounds__(256) void k_cnn_input_pair(const Stack fa, const Stack fb, int Hq,
+                                                        int Wq, float in_scale,
+                                                        uint16_t* __restrict__ x0a,
+                                                        uint16_t* __restrict__ x0b) {
+    static_assert(CNN_DS == 4, "one 4-sample load per cell row");
+    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+    if (X >= Wq)
+        return;
+    const Stack& f = blockIdx.z ? fb : fa;
+    float q[CNN_IN], m = 0.0f;
+#pragma unroll
+    for (int t = 0; t < CNN_IN; ++t) {
+        float s = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < CNN_DS; ++dy) {
+            const TIn* r = frame_row<TIn>(f, t, Y * CNN_DS + dy) + X * CNN_DS;
+            TIn v[4];
+            if (VEC) {
+                if (sizeof(TIn) == 2) {
+                    const uint2 w = *(const uint2*)r;
+                    v[0] = (TIn)(w.x & 0xFFFF); v[1] = (TIn)(w.x >> 16);
+                    v[2] = (TIn)(w.y & 0xFFFF); v[3] = (TIn)(w.y >> 16);
+                } else {
+                    const uint32_t w = *(const uint32_t*)r;
+                    v[0] = (TIn)(w & 0xFF); v[1] = (TIn)((w >> 8) & 0xFF);
+                    v[2] = (TIn)((w >> 16) & 0xFF); v[3] = (TIn)(w >> 24);
+                }
+            } else {
+#pragma unroll
+                for (int dx = 0; dx < 4; ++dx) v[dx] = r[dx];
+            }
+#pragma unroll
+            for (int dx = 0; dx < CNN_DS; ++dx) s += (float)v[dx];
+        }
+        q[t] = s * (1.0f / (CNN_DS * CNN_DS));
+        m += q[t];
+    }
+    m *= 1.0f / CNN_IN;
+    static_assert(CNN_IN == 4, "one 8-byte store per cell");
+    uint16_t h[CNN_IN];
+#pragma unroll
+    for (int t = 0; t < CNN_IN; ++t) h[t] = bf16_bits((q[t] - m) * in_scale);
+    uint16_t* x0 = blockIdx.z ? x0b : x0a;
+    *(uint2*)(x0 + ((size_t)Y * Wq + X) * CNN_IN) =
+        make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+}
+
 // BF16 copy of the weights (biases stay FP32)
 __global__ void k_cnn_bf16_weights(const float* __restrict__ p, float* __restrict__ q) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -352,15 +404,21 @@ __global__ __launch_bounds__(256) void k_cnn_window(const float* __restrict__ C,
 
 /* ------------------------------------------------------------ inference */
 
-// Inference weights: layer 0 as in the flat layout (BF16-rounded floats),
-// layers 1..3 as BF16 bits in [layer-1][co][tap][ci] so that one 16-byte
-// load gives 8 consecutive input channels of one tap.
+// Inference weights as BF16 bits: layer 0 in [co][k] with k = tap * 4 + ci
+// (36 values, zero-padded to CNN_K0 = 48: three MFMA steps of 16), layers
+// 1..3 in [layer-1][co][tap][ci] so that one 16-byte load gives 8
+// consecutive input channels of one tap.
+constexpr int CNN_K0 = 48;
+constexpr int CNN_MFMA_W0 = CNN_C * CNN_K0;
 constexpr int CNN_MFMA_W = 3 * CNN_C * 9 * CNN_C;
 
-__global__ void k_cnn_pack_mfma(const float* __restrict__ p, float* __restrict__ w0,
+__global__ void k_cnn_pack_mfma(const float* __restrict__ p, uint16_t* __restrict__ w0,
                                 uint16_t* __restrict__ wm) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < CNN_W1) w0[i] = bf16_round(p[cnn_w_off(0) + i]);
+    if (i < CNN_MFMA_W0) {
+        const int co = i / CNN_K0, k = i % CNN_K0, tap = k / CNN_IN, ci = k % CNN_IN;
+        w0[i] = tap < 9 ? bf16_bits(p[cnn_w_off(0) + (co * CNN_IN + ci) * 9 + tap]) : (uint16_t)0;
+    }
     if (i < CNN_MFMA_W) {
         const int l = i / (CNN_C * 9 * CNN_C), r = i % (CNN_C * 9 * CNN_C);
         const int co = r / (9 * CNN_C), tap = (r / CNN_C) % 9, ci = r % CNN_C;
@@ -368,48 +426,52 @@ __global__ void k_cnn_pack_mfma(const float* __restrict__ p, float* __restrict__
     }
 }
 
-// layer 0 (4 -> 32 channels, ReLU) on the vector ALU, from the planar x0
-// of view blockIdx.z to cell-major BF16
-__global__ __launch_bounds__(256) void k_cnn_conv0_cm(const float* __restrict__ x0a,
-                                                      const float* __restrict__ x0b,
-                                                      const float* __restrict__ w0,
-                                                      const float* __restrict__ b, int Hq, int Wq,
-                                                      uint16_t* __restrict__ ya,
-                                                      uint16_t* __restrict__ yb) {
-    __shared__ float ws[CNN_W1];
-    for (int i = threadIdx.x; i < CNN_W1; i += 256) ws[i] = w0[i];
-    __syncthreads();
-    const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
-    if (X >= Wq)
+// layer 0 (4 -> 32 channels, ReLU) on v_mfma_f32_32x32x16_bf16, from the
+// cell-major x0 of view blockIdx.z: one wave per 32-cell tile of a row,
+// D[co][cell] = W0[co][k] x In[k][cell] over k = tap * 4 + ci (three steps;
+// lane half g of step s holds taps 4 s + 2 g and 4 s + 2 g + 1). Output as
+// k_cnn_conv_mfma.
+__global__ __launch_bounds__(256) void k_cnn_conv0_mfma(const uint16_t* __restrict__ xa,
+                                                        const uint16_t* __restrict__ xb,
+                                                        const uint16_t* __restrict__ w0,
+                                                        const float* __restrict__ bias, int Hq,
+                                                        int Wq, uint16_t* __restrict__ ya,
+                                                        uint16_t* __restrict__ yb) {
+    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5;
+    const int tiles_row = (Wq + 31) / 32;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int Y = wave / tiles_row, X = (wave % tiles_row) * 32 + nl;
+    if (Y >= Hq)
         return;
-    const float* x = blockIdx.z ? x0b : x0a;
+    const uint16_t* x = blockIdx.z ? xb : xa;
     uint16_t* y = blockIdx.z ? yb : ya;
-    const size_t plane = (size_t)Hq * Wq;
-    float v[CNN_IN][9];
+    const auto tap_cell = [&](int tap) {  // 4 channels of the tap's cell, zero outside
+        uint2 v = make_uint2(0, 0);
+        const int yy = Y + tap / 3 - 1, xx = X + tap % 3 - 1;
+        if (tap < 9 && yy >= 0 && yy < Hq && xx >= 0 && xx < Wq)
+            v = *(const uint2*)(x + ((size_t)yy * Wq + xx) * CNN_IN);
+        return v;
+    };
+    f32x16 acc = {0};
 #pragma unroll
-    for (int ci = 0; ci < CNN_IN; ++ci)
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int yy = Y + k / 3 - 1, xx = X + k % 3 - 1;
-            v[ci][k] = (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq) ? x[ci * plane + (size_t)yy * Wq + xx] : 0.0f;
-        }
-    uint32_t packed[CNN_C / 2];
-#pragma unroll
-    for (int co = 0; co < CNN_C; co += 2) {
-        float a0 = b[co], a1 = b[co + 1];
-#pragma unroll
-        for (int ci = 0; ci < CNN_IN; ++ci)
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                a0 = __builtin_fmaf(ws[(co * CNN_IN + ci) * 9 + k], v[ci][k], a0);
-                a1 = __builtin_fmaf(ws[((co + 1) * CNN_IN + ci) * 9 + k], v[ci][k], a1);
-            }
-        packed[co / 2] = (uint32_t)bf16_bits(fmaxf(a0, 0.0f)) | ((uint32_t)bf16_bits(fmaxf(a1, 0.0f)) << 16);
+    for (int st = 0; st < CNN_K0 / 16; ++st) {
+        const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const uint4*)(w0 + nl * CNN_K0 + 16 * st + 8 * g));
+        const uint2 lo = tap_cell(4 * st + 2 * g), hi = tap_cell(4 * st + 2 * g + 1);
+        const bf16x8 bx = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa, bx, acc, 0, 0, 0);
     }
-    uint4* dst = (uint4*)(y + ((size_t)Y * Wq + X) * CNN_C);
+    if (X < Wq) {
+        uint16_t* dst = y + ((size_t)Y * Wq + X) * CNN_C + 4 * g;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+        for (int q = 0; q < 4; ++q) {
+            float o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                o[r] = fmaxf(acc[4 * q + r] + bias[8 * q + 4 * g + r], 0.0f);
+            *(uint2*)(dst + 8 * q) = make_uint2((uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16),
+                                                (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
+        }
+    }
 }
 
 // 16-byte load of 8 BF16 channels, zero outside the grid
@@ -420,13 +482,17 @@ __device__ __forceinline__ bf16x8 load_ch8(const uint16_t* t, int Hq, int Wq, in
     return __builtin_bit_cast(bf16x8, v);
 }
 
-// 3x3 conv, 32 -> 32 channels, on v_mfma_f32_32x32x16_bf16: a wave computes
-// tiles of 32 consecutive cells of one row as D[co][cell] = W[co][k] x
-// In[k][cell] with k = (tap, ci) (K = 288 = 18 steps of 16). The weight
-// operands stay in registers for the wave's CNN_TILES tiles. Output lane
-// layout: lane l, accumulator i -> cell l % 32, channel 8 (i / 4) +
-// 4 (l / 32) + i % 4, stored as 4 runs of 4 channels. View = blockIdx.z.
-constexpr int CNN_TILES = 1;
+// 3x3 conv, 32 -> 32 channels, on v_mfma_f32_32x32x16_bf16: D[co][cell] =
+// W[co][k] x In[k][cell] with k = (tap, ci) (K = 288 = 18 steps of 16). A
+// workgroup computes CNN_TROWS rows of one 32-cell tile column, a wave per
+// row: the input patch (rows and columns one beyond on each side, zero
+// outside the grid) is staged in LDS in one round of loads, issued together
+// with the wave's weight operands, and the B operands are read from there.
+// Output lane layout: lane l, accumulator i -> cell l % 32, channel 8 (i / 4)
+// + 4 (l / 32) + i % 4, stored as 4 runs of 4 channels. View = blockIdx.z.
+constexpr int CNN_TROWS = 4;
+constexpr int CNN_PCOLS = 34;    // patch columns
+constexpr int CNN_PSTRIDE = 40;  // patch cell stride in BF16 values (64 B of channels, 16 B pad)
 
 template<bool RELU>
 __global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restrict__ xa,
@@ -435,44 +501,56 @@ __global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restric
                                                        const float* __restrict__ bias, int Hq,
                                                        int Wq, uint16_t* __restrict__ ya,
                                                        uint16_t* __restrict__ yb) {
-    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5;
+    constexpr int PROWS = CNN_TROWS + 2, CHUNKS = PROWS * CNN_PCOLS * 4;  // 16-byte chunks
+    __shared__ uint4 patch_raw[PROWS * CNN_PCOLS * CNN_PSTRIDE / 8];
+    uint16_t* patch = (uint16_t*)patch_raw;
+    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5, wv = threadIdx.x >> 6;
     const int tiles_row = (Wq + 31) / 32;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int Y = wave / ((tiles_row + CNN_TILES - 1) / CNN_TILES);
-    const int t0 = (wave % ((tiles_row + CNN_TILES - 1) / CNN_TILES)) * CNN_TILES;
-    if (Y >= Hq)
-        return;
+    const int X0 = (blockIdx.x % tiles_row) * 32, Y0 = (blockIdx.x / tiles_row) * CNN_TROWS;
     const uint16_t* x = blockIdx.z ? xb : xa;
     uint16_t* y = blockIdx.z ? yb : ya;
     bf16x8 wa[18];
 #pragma unroll
     for (int s = 0; s < 18; ++s)
         wa[s] = __builtin_bit_cast(bf16x8, *(const uint4*)(wm + ((size_t)nl * 9 + (s >> 1)) * CNN_C + 16 * (s & 1) + 8 * g));
+#pragma unroll
+    for (int q = 0; q < (CHUNKS + 255) / 256; ++q) {
+        const int i = q * 256 + threadIdx.x;
+        if (i < CHUNKS) {
+            const int cell = i >> 2, part = i & 3, pr = cell / CNN_PCOLS, pc = cell % CNN_PCOLS;
+            const int yy = Y0 - 1 + pr, xx = X0 - 1 + pc;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq)
+                v = *(const uint4*)(x + ((size_t)yy * Wq + xx) * CNN_C + 8 * part);
+            *(uint4*)(patch + cell * CNN_PSTRIDE + 8 * part) = v;
+        }
+    }
     float bv[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) bv[i] = bias[8 * (i / 4) + 4 * g + (i % 4)];
-    for (int tt = t0; tt < min(t0 + CNN_TILES, tiles_row); ++tt) {
-        const int X = tt * 32 + nl;
-        f32x16 acc = {0};
+    __syncthreads();
+    const int Y = Y0 + wv, X = X0 + nl;
+    if (Y >= Hq)
+        return;
+    f32x16 acc = {0};
 #pragma unroll
-        for (int s = 0; s < 18; ++s) {
-            const int tap = s >> 1;
-            const bf16x8 bx = load_ch8(x, Hq, Wq, Y + tap / 3 - 1, X + tap % 3 - 1, 16 * (s & 1) + 8 * g);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bx, acc, 0, 0, 0);
-        }
-        if (X < Wq) {
-            uint16_t* dst = y + ((size_t)Y * Wq + X) * CNN_C + 4 * g;
+    for (int s = 0; s < 18; ++s) {
+        const int tap = s >> 1, ky = tap / 3, kx = tap % 3;
+        const bf16x8 bx = *(const bf16x8*)(patch + ((wv + ky) * CNN_PCOLS + nl + kx) * CNN_PSTRIDE + 16 * (s & 1) + 8 * g);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bx, acc, 0, 0, 0);
+    }
+    if (X < Wq) {
+        uint16_t* dst = y + ((size_t)Y * Wq + X) * CNN_C + 4 * g;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float o[4];
+        for (int q = 0; q < 4; ++q) {
+            float o[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float v = acc[4 * q + r] + bv[4 * q + r];
-                    o[r] = RELU ? fmaxf(v, 0.0f) : v;
-                }
-                *(uint2*)(dst + 8 * q) = make_uint2((uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16),
-                                                    (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
+            for (int r = 0; r < 4; ++r) {
+                const float v = acc[4 * q + r] + bv[4 * q + r];
+                o[r] = RELU ? fmaxf(v, 0.0f) : v;
             }
+            *(uint2*)(dst + 8 * q) = make_uint2((uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16),
+                                                (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
         }
     }
 }
@@ -485,21 +563,25 @@ __global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restric
 // k_cnn_window does. K <= CNN_KMAX_MFMA.
 constexpr int CNN_KMAX_MFMA = 160;
 
+// dynamic LDS of k_cnn_corr_window for K bins
+inline size_t cnn_corr_window_lds(int K) { return (size_t)4 * 32 * (K + 1) * sizeof(float); }
+
 __global__ __launch_bounds__(256) void k_cnn_corr_window(const uint16_t* __restrict__ fl,
                                                          const uint16_t* __restrict__ fr, int Hq,
                                                          int Wq, int K, float p_conf, int hw_lo,
                                                          int hw_hi, float* __restrict__ center,
                                                          int16_t* __restrict__ halfwidth,
                                                          float* __restrict__ peak_prob) {
-    __shared__ float sc[4][32][CNN_KMAX_MFMA + 1];
+    extern __shared__ float sc_dyn[];
     const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5, wv = threadIdx.x >> 6;
     const int tiles_row = (Wq + 31) / 32;
     const int wave = blockIdx.x * 4 + wv;
     const int Y = wave / tiles_row, X0 = (wave % tiles_row) * 32;
     if (Y >= Hq)
         return;
-    float (&S)[32][CNN_KMAX_MFMA + 1] = sc[wv];
-    for (int i = lane; i < 32 * K; i += 64) S[i / K][i % K] = -1e30f;
+    const int ld = K + 1;
+    float* S = sc_dyn + (size_t)wv * 32 * ld;  // S[cell * ld + k]
+    for (int i = lane; i < 32 * K; i += 64) S[(i / K) * ld + i % K] = -1e30f;
     // left operands: cell X0 + nl, channels 16 h + 8 g .. +7
     const bf16x8 a0 = load_ch8(fl, Hq, Wq, Y, X0 + nl, 8 * g);
     const bf16x8 a1 = load_ch8(fl, Hq, Wq, Y, X0 + nl, 16 + 8 * g);
@@ -518,31 +600,43 @@ __global__ __launch_bounds__(256) void k_cnn_corr_window(const uint16_t* __restr
             for (int i = 0; i < 16; ++i) {
                 const int m = 8 * (i / 4) + 4 * g + (i % 4);
                 const int k = (X0 + m) - Xr - CNN_D0;
-                if (k >= 0 && k < K) S[m][k] = acc[i] * 0.17677669529663687f;  // 1 / sqrt(32)
+                if (k >= 0 && k < K) S[m * ld + k] = acc[i] * 0.17677669529663687f;  // 1 / sqrt(32)
             }
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int X = X0 + lane;
-    if (lane >= 32 || X >= Wq)
-        return;
-    int kb = 0;
+    // cell nl: lane half g scans bins [g * Kh, min(K, (g + 1) * Kh))
+    const int Kh = (K + 1) / 2, k0 = g * Kh, k1 = min(K, k0 + Kh);
+    const float* Sr = S + nl * ld;
+    int kb = k0;
     float mx = -1e30f;
-    for (int k = 0; k < K; ++k) {
-        const float c = S[lane][k];
+    for (int k = k0; k < k1; ++k) {
+        const float c = Sr[k];
         if (c > mx) { mx = c; kb = k; }
     }
+    // first maximum over both halves: the upper half's only if larger
+    const float omx = __shfl_xor(mx, 32);
+    const int okb = __shfl_xor(kb, 32);
+    const float lmx = g ? omx : mx, umx = g ? mx : omx;
+    const int lkb = g ? okb : kb, ukb = g ? kb : okb;
+    const bool up = umx > lmx;
+    mx = up ? umx : lmx;
+    kb = up ? ukb : lkb;
     float z = 0.0f;
-    for (int k = 0; k < K; ++k) {
-        const float c = S[lane][k];
+    for (int k = k0; k < k1; ++k) {
+        const float c = Sr[k];
         z += c > -1e29f ? __expf(c - mx) : 0.0f;
     }
+    z += __shfl_xor(z, 32);
+    const int X = X0 + nl;
+    if (g || X >= Wq)
+        return;
     const float pk = 1.0f / z;
     float kf = (float)kb;
     if (kb > 0 && kb < K - 1) {
-        const float cm = S[lane][kb - 1], cp = S[lane][kb + 1];
+        const float cm = Sr[kb - 1], cp = Sr[kb + 1];
         if (cm > -1e29f && cp > -1e29f) {
             const float den = cm - 2.0f * mx + cp;
             if (den < 0.0f) kf += fminf(fmaxf(0.5f * (cm - cp) / den, -0.5f), 0.5f);

@@ -149,8 +149,8 @@ size_t valu_scratch_bytes(const Grid& g) {
 }
 
 size_t mfma_scratch_bytes(const Grid& g) {
-    return align_up(CNN_W1 * 4, 256) + align_up(CNN_MFMA_W * 2, 256) +
-           2 * align_up(CNN_IN * g.cells * 4, 256) + 4 * align_up(CNN_C * g.cells * 2, 256);
+    return align_up(CNN_MFMA_W0 * 2, 256) + align_up(CNN_MFMA_W * 2, 256) +
+           2 * align_up(CNN_IN * g.cells * 2, 256) + 4 * align_up(CNN_C * g.cells * 2, 256);
 }
 
 size_t windows_scratch_bytes(const Grid& g) {
@@ -167,27 +167,34 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
         if (g.K > CNN_KMAX_MFMA)
             return fail(BICOS_ERR_INVALID_ARGUMENT, "window bins above 160 need the vector-ALU network");
         uint8_t* p = (uint8_t*)scratch;
-        float* w0 = (float*)p; p += align_up(CNN_W1 * 4, 256);
+        uint16_t* w0 = (uint16_t*)p; p += align_up(CNN_MFMA_W0 * 2, 256);
         uint16_t* wm = (uint16_t*)p; p += align_up(CNN_MFMA_W * 2, 256);
-        float* x0L = (float*)p; p += align_up(CNN_IN * g.cells * 4, 256);
-        float* x0R = (float*)p; p += align_up(CNN_IN * g.cells * 4, 256);
+        uint16_t* x0L = (uint16_t*)p; p += align_up(CNN_IN * g.cells * 2, 256);
+        uint16_t* x0R = (uint16_t*)p; p += align_up(CNN_IN * g.cells * 2, 256);
         uint16_t* aL = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
         uint16_t* bL = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
         uint16_t* aR = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
         uint16_t* bR = (uint16_t*)p;
         LAUNCH("cnn_weights", k_cnn_pack_mfma, dim3((CNN_MFMA_W + 255) / 256), block, 0, s, params, w0, wm);
+        // one 4-sample load per cell row where both stacks allow it
+        const size_t vb = 4 * (depth == BICOS_8U ? 1 : 2);
+        const auto aligned = [&](const Stack& v) {
+            return ((uintptr_t)v.base % vb) == 0 && ((size_t)v.fstride % vb) == 0 && ((size_t)v.step % vb) == 0;
+        };
+        const bool vec = aligned(v0) && aligned(v1);
+        const dim3 igrid((g.Wq + 255) / 256, g.Hq, 2);
+#define CI(T, V) LAUNCH("cnn_input", (k_cnn_input_pair<T, V>), igrid, block, 0, s, v0, v1, g.Hq, g.Wq, in_scale, x0L, x0R)
         if (depth == BICOS_8U) {
-            LAUNCH("cnn_input", k_cnn_input<uint8_t>, grid, block, 0, s, v0, g.W, g.Hq, g.Wq, in_scale, x0L);
-            LAUNCH("cnn_input", k_cnn_input<uint8_t>, grid, block, 0, s, v1, g.W, g.Hq, g.Wq, in_scale, x0R);
+            if (vec) CI(uint8_t, true); else CI(uint8_t, false);
         } else {
-            LAUNCH("cnn_input", k_cnn_input<uint16_t>, grid, block, 0, s, v0, g.W, g.Hq, g.Wq, in_scale, x0L);
-            LAUNCH("cnn_input", k_cnn_input<uint16_t>, grid, block, 0, s, v1, g.W, g.Hq, g.Wq, in_scale, x0R);
+            if (vec) CI(uint16_t, true); else CI(uint16_t, false);
         }
-        LAUNCH("cnn_conv", k_cnn_conv0_cm, dim3((g.Wq + 255) / 256, g.Hq, 2), block, 0, s,
-               (const float*)x0L, (const float*)x0R, (const float*)w0, params + cnn_b_off(0), g.Hq, g.Wq, aL, aR);
+#undef CI
         const int tiles_row = (g.Wq + 31) / 32;
-        const int waves = g.Hq * ((tiles_row + CNN_TILES - 1) / CNN_TILES);
-        const dim3 cgrid((waves + 3) / 4, 1, 2);
+        LAUNCH("cnn_conv0", k_cnn_conv0_mfma, dim3((g.Hq * tiles_row + 3) / 4, 1, 2), block, 0, s,
+               (const uint16_t*)x0L, (const uint16_t*)x0R, (const uint16_t*)w0, params + cnn_b_off(0),
+               g.Hq, g.Wq, aL, aR);
+        const dim3 cgrid(tiles_row * ((g.Hq + CNN_TROWS - 1) / CNN_TROWS), 1, 2);
         const size_t wl = (size_t)CNN_C * 9 * CNN_C;
         LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, block, 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
                (const uint16_t*)wm, params + cnn_b_off(1), g.Hq, g.Wq, bL, bR);
@@ -196,7 +203,11 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
         LAUNCH("cnn_conv", k_cnn_conv_mfma<false>, cgrid, block, 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
                (const uint16_t*)(wm + 2 * wl), params + cnn_b_off(3), g.Hq, g.Wq, bL, bR);
         const int cw = g.Hq * tiles_row;
-        LAUNCH("cnn_corr_window", k_cnn_corr_window, dim3((cw + 3) / 4), block, 0, s, (const uint16_t*)bL,
+        const size_t clds = cnn_corr_window_lds(g.K);
+        if (clds > 65536)  // above the default limit of dynamic LDS per workgroup
+            HIP_TRY(hipFuncSetAttribute((const void*)k_cnn_corr_window,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
+        LAUNCH("cnn_corr_window", k_cnn_corr_window, dim3((cw + 3) / 4), block, clds, s, (const uint16_t*)bL,
                (const uint16_t*)bR, g.Hq, g.Wq, g.K, o->p_conf, o->hw_lo, o->hw_hi, center, halfwidth, peak);
         return BICOS_OK;
     }
