@@ -198,7 +198,14 @@ int run_transform(const Stack& f0, const Stack* f1, uint32_t n, int mode, int W,
                 case 1: LAUNCH("transform", (k_transform_limited_v4<TIn, 1, 9>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
                 case 2: LAUNCH("transform", (k_transform_limited_v4<TIn, 2, 17>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
                 case 4: LAUNCH("transform", (k_transform_limited_v4<TIn, 4, 33>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
-                case 8: LAUNCH("transform", (k_transform_limited_v4<TIn, 8, 65>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
+                case 8:
+                    if constexpr (sizeof(TIn) == 2) {  // 2 pixels per lane (register budget of 65 frames)
+                        const dim3 grid2((W + 511) / 512, H, views);
+                        LAUNCH("transform", (k_transform_limited_v4<TIn, 8, 65, 2>), grid2, block, 0, s, f0, g1, (int)n, W, words, out0, out1);
+                    } else {
+                        LAUNCH("transform", (k_transform_limited_v4<TIn, 8, 65>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1);
+                    }
+                    break;
                 default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
             }
         } else {

@@ -181,13 +181,15 @@ __device__ __forceinline__ void load_px4(const Stack& f, int t, int row, int col
     }
 }
 
-// 4 adjacent raw samples of frame t (zero beyond the row end)
-template<typename TIn>
-__device__ __forceinline__ void load_raw4(const Stack& f, int t, int row, int col, int W,
-                                          uint32_t (&r)[TPX * sizeof(TIn) / 4]) {
+// PX adjacent raw samples of frame t packed into 32-bit words (zero beyond
+// the row end); PX * sizeof(TIn) is 4 or 8 bytes
+template<typename TIn, int PX>
+__device__ __forceinline__ void load_raw(const Stack& f, int t, int row, int col, int W,
+                                         uint32_t (&r)[PX * sizeof(TIn) / 4]) {
+    static_assert(PX * sizeof(TIn) == 4 || PX * sizeof(TIn) == 8, "4- or 8-byte groups");
     const TIn* p = frame_row<TIn>(f, t, row) + col;
-    if (col + TPX <= W) {
-        if constexpr (sizeof(TIn) == 2) {
+    if (col + PX <= W) {
+        if constexpr (PX * sizeof(TIn) == 8) {
             const uint2 u = *(const uint2*)p;
             r[0] = u.x;
             r[1] = u.y;
@@ -195,27 +197,23 @@ __device__ __forceinline__ void load_raw4(const Stack& f, int t, int row, int co
             r[0] = *(const uint32_t*)p;
         }
     } else {
-        TIn v[TPX];
 #pragma unroll
-        for (int k = 0; k < TPX; ++k) v[k] = col + k < W ? p[k] : (TIn)0;
-        if constexpr (sizeof(TIn) == 2) {
-            r[0] = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-            r[1] = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-        } else {
-            r[0] = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
-        }
+        for (int j = 0; j < (int)(PX * sizeof(TIn) / 4); ++j) r[j] = 0u;
+#pragma unroll
+        for (int k = 0; k < PX; ++k)
+            if (col + k < W) {
+                constexpr int PER = 4 / sizeof(TIn);
+                r[k / PER] |= (uint32_t)p[k] << (8 * sizeof(TIn) * (k % PER));
+            }
     }
 }
 
-template<typename TIn>
-__device__ __forceinline__ void unpack4(const uint32_t (&r)[TPX * sizeof(TIn) / 4], float (&v)[TPX]) {
-    if constexpr (sizeof(TIn) == 2) {
-        v[0] = (float)(r[0] & 0xFFFFu); v[1] = (float)(r[0] >> 16);
-        v[2] = (float)(r[1] & 0xFFFFu); v[3] = (float)(r[1] >> 16);
-    } else {
-        v[0] = (float)(r[0] & 0xFFu); v[1] = (float)((r[0] >> 8) & 0xFFu);
-        v[2] = (float)((r[0] >> 16) & 0xFFu); v[3] = (float)(r[0] >> 24);
-    }
+template<typename TIn, int PX>
+__device__ __forceinline__ void unpack(const uint32_t (&r)[PX * sizeof(TIn) / 4], float (&v)[PX]) {
+    constexpr int PER = 4 / sizeof(TIn);
+    constexpr uint32_t MASK = sizeof(TIn) == 2 ? 0xFFFFu : 0xFFu;
+#pragma unroll
+    for (int k = 0; k < PX; ++k) v[k] = (float)((r[k / PER] >> (8 * sizeof(TIn) * (k % PER))) & MASK);
 }
 
 // shift the comparison x < y into the low end of g
@@ -231,22 +229,24 @@ __device__ __forceinline__ void put_group(uint32_t (&w)[WORDS], int pos, int len
         w[(pos >> 5) + 1] |= g >> (32 - (pos & 31));
 }
 
-template<int WORDS>
+template<int WORDS, int PX = TPX>
 __device__ __forceinline__ void store_px(uint32_t* __restrict__ out, int W, int row, int col,
-                                         const uint32_t (&w)[TPX][WORDS], int out_words) {
+                                         const uint32_t (&w)[PX][WORDS], int out_words) {
 #pragma unroll
-    for (int k = 0; k < TPX; ++k)
+    for (int k = 0; k < PX; ++k)
         if (col + k < W)
             store_desc(out + ((size_t)row * W + col + k) * out_words, w[k], out_words);
 }
 
-// limited mode, runtime n <= NMAX (descriptor_transform.hpp:31-73)
-template<typename TIn, int WORDS, int NMAX>
+// limited mode, runtime n <= NMAX (descriptor_transform.hpp:31-73); PX
+// pixels per lane (4, or 2 for deep 16-bit stacks, where holding 4 pixels'
+// frames would cost the occupancy that hides the memory latency)
+template<typename TIn, int WORDS, int NMAX, int PX = TPX>
 __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, const Stack f1,
                                                               int n, int W, int out_words,
                                                               uint32_t* __restrict__ out0,
                                                               uint32_t* __restrict__ out1) {
-    const int col = (blockIdx.x * 256 + threadIdx.x) * TPX, row = blockIdx.y;
+    const int col = (blockIdx.x * 256 + threadIdx.x) * PX, row = blockIdx.y;
     if (col >= W)
         return;
     const Stack& f = blockIdx.z ? f1 : f0;
@@ -256,38 +256,40 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
     // register for 16-bit input): the mean first, then the comparisons over
     // a sliding window of three unpacked frames. Half the registers of
     // holding floats, so more waves hide the memory latency.
-    uint32_t raw[NMAX][TPX * sizeof(TIn) / 4];
-    float av[TPX] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t raw[NMAX][PX * sizeof(TIn) / 4];
+    float av[PX];
+#pragma unroll
+    for (int k = 0; k < PX; ++k) av[k] = 0.0f;
 #pragma unroll
     for (int t = 0; t < NMAX; ++t) {
         if (t < n) {
-            load_raw4<TIn>(f, t, row, col, W, raw[t]);
-            float v[TPX];
-            unpack4<TIn>(raw[t], v);
+            load_raw<TIn, PX>(f, t, row, col, W, raw[t]);
+            float v[PX];
+            unpack<TIn, PX>(raw[t], v);
 #pragma unroll
-            for (int k = 0; k < TPX; ++k) av[k] += v[k];
+            for (int k = 0; k < PX; ++k) av[k] += v[k];
         }
     }
 #pragma unroll
-    for (int k = 0; k < TPX; ++k) av[k] /= (float)n;
+    for (int k = 0; k < PX; ++k) av[k] /= (float)n;
 
-    uint32_t w[TPX][WORDS];
+    uint32_t w[PX][WORDS];
 #pragma unroll
-    for (int k = 0; k < TPX; ++k)
+    for (int k = 0; k < PX; ++k)
 #pragma unroll
         for (int j = 0; j < WORDS; ++j) w[k][j] = 0u;
 
-    float pa[TPX], pb[TPX], pc[TPX];  // frames t, t + 1, t + 2
-    unpack4<TIn>(raw[0], pa);
-    unpack4<TIn>(raw[1], pb);
-    float prev0[TPX], prev1[TPX];
+    float pa[PX], pb[PX], pc[PX];  // frames t, t + 1, t + 2
+    unpack<TIn, PX>(raw[0], pa);
+    unpack<TIn, PX>(raw[1], pb);
+    float prev0[PX], prev1[PX];
 #pragma unroll
     for (int t = 0; t < NMAX - 2; ++t) {
         if (t < n - 2) {
             const int pos = t < 2 ? 3 * t : 4 * t - 2;
-            unpack4<TIn>(raw[t + 2], pc);
+            unpack<TIn, PX>(raw[t + 2], pc);
 #pragma unroll
-            for (int k = 0; k < TPX; ++k) {
+            for (int k = 0; k < PX; ++k) {
                 const float a = pa[k], b = pb[k], c = pc[k];
                 const float cur = a + b;
                 uint32_t g = 0;
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
                 if (t & 1) prev1[k] = cur; else prev0[k] = cur;
             }
 #pragma unroll
-            for (int k = 0; k < TPX; ++k) {
+            for (int k = 0; k < PX; ++k) {
                 pa[k] = pb[k];
                 pb[k] = pc[k];
             }
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
     const int P = n >= 4 ? 4 * n - 10 : 3 * (n - 2);
     const int wi = P >> 5, sh = P & 31;
 #pragma unroll
-    for (int k = 0; k < TPX; ++k) {
+    for (int k = 0; k < PX; ++k) {
         const float a = pa[k], b = pb[k];
         const float prevf = n < 4 ? -1.0f : (((n - 2) & 1) ? prev1[k] : prev0[k]);
         uint32_t g = 0;
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
             if (j == wi + 1 && sh > 28) w[k][j] |= g >> (32 - sh);
         }
     }
-    store_px(out, W, row, col, w, out_words);
+    store_px<WORDS, PX>(out, W, row, col, w, out_words);
 }
 
 // full mode, compile-time N (descriptor_transform.hpp:75-123); the whole
