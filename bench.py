@@ -61,7 +61,7 @@ def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int
                 "unit": "TFLOP/s", "frac": ops / s / 1e12 / INT8_DENSE_TOPS,
                 "work_per_launch": ops, "work_unit": "int8-equivalent ops"}
     if name.startswith("transform"):
-        b = W * H * (n * 2 + 4 * words)
+        b = 2 * W * H * (n * 2 + 4 * words)  # one launch transforms both views
     elif name.startswith("agree_subpixel"):
         b = W * H * (2 + 4 * n * 2 + 4)      # disparity in, 4n samples, float out
     else:

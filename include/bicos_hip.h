@@ -204,8 +204,7 @@ typedef struct bicos_window_options {
  *   ALL     all of them: a pixel the windowed path rejects is then one the
  *           plain path rejects too;
  *   UNSURE  those in cells the network was unsure about (peak probability
- *           below p_conf: the cells given the hw_hi window), and the pixels
- *           the MOMENTS stage leaves undecided. */
+ *           below p_conf: the cells given the hw_hi window). */
 enum { BICOS_FALLBACK_NONE = 0, BICOS_FALLBACK_ALL = 1, BICOS_FALLBACK_UNSURE = 2 };
 
 typedef struct bicos_window_stats {
