@@ -310,17 +310,18 @@ int run_search_list_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W,
     return BICOS_OK;
 }
 
-// Full-row search of the listed pixels (same kernel shapes as the full
-// search of bicos_hip_match).
+// Full-row search of the listed pixels: the full search's kernels with one
+// 32-query tile per wave (a row lists a tenth of its pixels or so, and
+// smaller query groups keep more of a workgroup's waves busy).
 int run_search_list(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
                     int mode, const int16_t* qcol, const int* row_count, int16_t* res,
                     int16_t* disp, int64_t dstep, hipStream_t s) {
     if (g_search_impl == BICOS_SEARCH_MFMA) {
         switch (words) {
-            case 1: return run_search_list_mfma_w<1, 2, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
-            case 2: return run_search_list_mfma_w<2, 2, 8, 800>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
-            case 4: return run_search_list_mfma_w<4, 2, 8, 448>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
-            case 8: return run_search_list_mfma_w<8, 2, 8, 224>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 1: return run_search_list_mfma_w<1, 1, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 2: return run_search_list_mfma_w<2, 1, 8, 800>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 4: return run_search_list_mfma_w<4, 1, 8, 448>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 8: return run_search_list_mfma_w<8, 1, 8, 224>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
     }
