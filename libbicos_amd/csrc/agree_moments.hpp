@@ -232,31 +232,33 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     double su = 0, suu = 0;
     float s0 = 0, s1 = 0, s2 = 0, su0 = 0, su1 = 0, su2 = 0;
     float s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0, su_f = 0;
-    float ymin = 3.0e38f, ymax = -3.0e38f;
+    int ylo = 0x7FFFFFFF, yhi = -0x7FFFFFFF;
     // branch-free loop (the right samples of lanes without an interior match
     // are read at a valid column and ignored), so that loads of many frames
     // are in flight at once
-    const float u_ref = (float)load_sample_cached<TIn>(f0, 0, row, col);
-    const float y_ref0 = (float)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
-    const float y_ref1 = (float)load_sample_cached<TIn>(f1, 0, row, c1);
-    const float y_ref2 = (float)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
+    const int u_ref = (int)load_sample_cached<TIn>(f0, 0, row, col);
+    const int y_ref0 = (int)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
+    const int y_ref1 = (int)load_sample_cached<TIn>(f1, 0, row, c1);
+    const int y_ref2 = (int)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
 #pragma unroll 11
     for (int t = 0; t < NMAX; ++t)
         if (n.has(t)) {
-            const float uf = (float)load_sample_cached<TIn>(f0, t, row, col);
-            const float f0y = (float)load_sample_cached<TIn>(f1, t, row, c1 - 1);
-            const float f1y = (float)load_sample_cached<TIn>(f1, t, row, c1);
-            const float f2y = (float)load_sample_cached<TIn>(f1, t, row, c1 + 1);
-            const double u = uf;
+            const int ui = (int)load_sample_cached<TIn>(f0, t, row, col);
+            const int i0 = (int)load_sample_cached<TIn>(f1, t, row, c1 - 1);
+            const int i1 = (int)load_sample_cached<TIn>(f1, t, row, c1);
+            const int i2 = (int)load_sample_cached<TIn>(f1, t, row, c1 + 1);
+            const double u = (double)ui;
             su += u;
             suu = __builtin_fma(u, u, suu);
-            // reach of the frame's parabola on [-1, 1]: the end values and,
-            // where it is convex (concave) with the vertex inside, the
-            // vertex, which lies above (below) y1 -/+ |y2 - y0| / 4
-            const float r4 = 0.25f * fabsf(f2y - f0y);
-            ymin = fminf(ymin, fminf(fminf(f0y, f2y), f1y - r4));
-            ymax = fmaxf(ymax, fmaxf(fmaxf(f0y, f2y), f1y + r4));
-            const float us = uf - u_ref, y0 = f0y - y_ref0, y1 = f1y - y_ref1, y2 = f2y - y_ref2;
+            // reach of the frame's parabola on [-1, 1], in integers: the end
+            // values and, where it is convex (concave) with the vertex
+            // inside, the vertex, which lies above (below) y1 -/+ |y2 - y0| / 4
+            // (rounded up here)
+            const int r4 = (abs(i2 - i0) + 3) >> 2;
+            ylo = min(ylo, min(min(i0, i2), i1 - r4));
+            yhi = max(yhi, max(max(i0, i2), i1 + r4));
+            const float us = (float)(ui - u_ref), y0 = (float)(i0 - y_ref0), y1 = (float)(i1 - y_ref1),
+                        y2 = (float)(i2 - y_ref2);
             su_f += us;
             s0 += y0; s1 += y1; s2 += y2;
             su0 = __builtin_fmaf(us, y0, su0); su1 = __builtin_fmaf(us, y1, su1);
@@ -312,8 +314,8 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             const float bc = (float)(0.5 * (g12 - g01));
             const float cc = (float)g11;
             // Bound on what the left-out rounding can change. If every
-            // frame's parabola stays inside [1, max(TIn) - 1] (with 0.1 for
-            // the FP32 evaluation and the approximate vertex), the cast of
+            // frame's parabola stays inside [2, max(TIn) - 2] (the integer
+            // reach above; the margin covers the FP32 evaluation), the cast of
             // agree.hpp:164 is the identity and every interpolated sample
             // moves by at most 0.51 (0.5 plus the FP32 evaluation error):
             // |e| <= delta = 0.51 sqrt(n). Then with s = sqrt(var1),
@@ -326,17 +328,28 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             const float delta = 0.51f * __builtin_sqrtf((float)n.get());
             float best_x = 0.0f, best = -1.0f, eps_best = 2.0f;
             float upper = -1.0f;  // max over x of closed form + bound
-            bool settled = ymin >= 1.1f && ymax <= VMAX - 1.1f;  // (NaN-safe: NaN fails)
+            bool settled = ylo >= 2 && yhi <= (int)VMAX - 2;
             const float rv0 = __builtin_amdgcn_rsqf((float)var0d);
+            // The closed form is evaluated with fused multiply-adds (its FP32
+            // error is covered by the 1e-4 above); this file is otherwise
+            // compiled without contraction for the reference's arithmetic.
+            const float ab2 = 2.0f * ab, bbac = bb + 2.0f * ac, bc2 = 2.0f * bc;
+            const float dd2 = 2.0f * delta, dsq1 = __builtin_fmaf(delta, delta, 1.0f);
+            const auto cov_at = [&](float x) {
+                return __builtin_fmaf(__builtin_fmaf(ua, x, ub), x, uc);
+            };
+            const auto v1_at = [&](float x) {
+                return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(__builtin_fmaf(aa, x, ab2), x, bbac), x, bc2), x, cc);
+            };
             const auto bound = [&](float sv) {
-                return 2.000001f * delta * __builtin_amdgcn_rcpf(sv - delta) + 1e-4f;
+                return __builtin_fmaf(2.000001f * delta, __builtin_amdgcn_rcpf(sv - delta), 1e-4f);
             };
             for (float x = -1.0f; x <= 1.0f; x += step) {
-                const float cov = (ua * x + ub) * x + uc;
-                const float v1 = (((aa * x + 2.0f * ab) * x + (bb + 2.0f * ac)) * x + 2.0f * bc) * x + cc;
+                const float cov = cov_at(x);
+                const float v1 = v1_at(x);
                 const float r1 = __builtin_amdgcn_rsqf(v1);
                 const float sv = v1 * r1;  // sqrt(var1)
-                const float margin = 2.0f * sv * delta + delta * delta + 1e-4f * v1 + 1.0f;
+                const float margin = __builtin_fmaf(dd2, sv, __builtin_fmaf(1e-4f, v1, dsq1));
                 float nxc = -1.0f, eps = 0.0f;
                 if (mv && v1 < minvar) {
                     // below the variance test: also exactly when below by the margin
@@ -361,11 +374,10 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                 float far = -1.0f;
                 for (float x = -1.0f; x <= 1.0f; x += step) {
                     if (fabsf(x - best_x) < 0.5f) continue;
-                    const float cov = (ua * x + ub) * x + uc;
-                    const float v1 = (((aa * x + 2.0f * ab) * x + (bb + 2.0f * ac)) * x + 2.0f * bc) * x + cc;
+                    const float v1 = v1_at(x);
                     const float r1 = __builtin_amdgcn_rsqf(v1);
                     if (mv && v1 < minvar) continue;  // settled: exactly -1 there
-                    far = fmaxf(far, cov * r1 * rv0 + bound(v1 * r1));
+                    far = fmaxf(far, __builtin_fmaf(cov_at(x) * r1, rv0, bound(v1 * r1)));
                 }
                 if (far < best - eps_best)
                     res = (float)d - best_x;
