@@ -216,11 +216,12 @@ int bicos_hip_window_param_count(void);
 void bicos_hip_window_options_default(bicos_window_options* opts);
 
 /* Network implementation of predict/match_windowed for subsequent calls:
- * BICOS_CNN_MFMA (default; cell-major BF16 tensors, 3x3x32 layers and the
- * correlation on v_mfma_f32_32x32x16_bf16, bins <= 160) or BICOS_CNN_VALU
- * (the training kernels). Both compute the same network up to FP32
- * summation order. */
-enum { BICOS_CNN_MFMA = 0, BICOS_CNN_VALU = 1 };
+ * BICOS_CNN_MFMA (default; cell-major BF16 tensors, the four layers in one
+ * kernel and the correlation on v_mfma_f32_32x32x16_bf16, bins <= 160),
+ * BICOS_CNN_MFMA_LAYERS (the same products, one kernel per layer; equal
+ * results bit for bit) or BICOS_CNN_VALU (the training kernels). All compute
+ * the same network up to FP32 summation order. */
+enum { BICOS_CNN_MFMA = 0, BICOS_CNN_VALU = 1, BICOS_CNN_MFMA_LAYERS = 2 };
 int bicos_hip_set_cnn_impl(int32_t impl);
 
 /* Deterministic He-normal initial parameters (biases 0), host memory. */

@@ -491,11 +491,12 @@ __device__ __forceinline__ bf16x8 load_ch8(const uint16_t* t, int Hq, int Wq, in
 // Output lane layout: lane l, accumulator i -> cell l % 32, channel 8 (i / 4)
 // + 4 (l / 32) + i % 4, stored as 4 runs of 4 channels. View = blockIdx.z.
 constexpr int CNN_TROWS = 4;
+constexpr int CNN_CONV_THREADS = CNN_TROWS * 64;
 constexpr int CNN_PCOLS = 34;    // patch columns
 constexpr int CNN_PSTRIDE = 40;  // patch cell stride in BF16 values (64 B of channels, 16 B pad)
 
 template<bool RELU>
-__global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restrict__ xa,
+__global__ __launch_bounds__(CNN_CONV_THREADS) void k_cnn_conv_mfma(const uint16_t* __restrict__ xa,
                                                        const uint16_t* __restrict__ xb,
                                                        const uint16_t* __restrict__ wm,
                                                        const float* __restrict__ bias, int Hq,
@@ -514,8 +515,8 @@ __global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restric
     for (int s = 0; s < 18; ++s)
         wa[s] = __builtin_bit_cast(bf16x8, *(const uint4*)(wm + ((size_t)nl * 9 + (s >> 1)) * CNN_C + 16 * (s & 1) + 8 * g));
 #pragma unroll
-    for (int q = 0; q < (CHUNKS + 255) / 256; ++q) {
-        const int i = q * 256 + threadIdx.x;
+    for (int q = 0; q < (CHUNKS + CNN_CONV_THREADS - 1) / CNN_CONV_THREADS; ++q) {
+        const int i = q * CNN_CONV_THREADS + threadIdx.x;
         if (i < CHUNKS) {
             const int cell = i >> 2, part = i & 3, pr = cell / CNN_PCOLS, pc = cell % CNN_PCOLS;
             const int yy = Y0 - 1 + pr, xx = X0 - 1 + pc;
@@ -553,6 +554,152 @@ __global__ __launch_bounds__(256) void k_cnn_conv_mfma(const uint16_t* __restric
                                                 (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
         }
     }
+}
+
+// The four layers in one kernel: a workgroup of 8 waves computes the
+// features of a CNN_FT_H x CNN_FT_W tile of cells. The tile's input (4-cell
+// halo) is staged in LDS once, and each layer's activations over the region
+// the later layers need (1 cell less per side per layer) stay in LDS: a1 in
+// buffer A, a2 in B, a3 in A again; only the features go to memory. Each
+// region row is one 32-cell MFMA row tile (the regions are at most 32
+// cells wide; lanes past a region's width compute values that are not
+// stored). Every output takes the same products in the same order as
+// k_cnn_conv0_mfma / k_cnn_conv_mfma, with the same bias, ReLU and BF16
+// rounding, and cells outside the grid are stored as zero (the layers' zero
+// padding), so the features equal the per-layer kernels' bit for bit.
+constexpr int CNN_FT_W = 26, CNN_FT_H = 8;  // feature cells per workgroup
+constexpr int CNN_FT_COLS = 34;             // stored cells per region row (32 + 2 read past)
+constexpr int CNN_FT_STRIDE = 40;           // BF16 values per stored cell (32 channels, 16 B pad)
+constexpr int CNN_FT_THREADS = 512;
+
+// one 3x3x32 layer over the stored region `in` (origin one cell up-left of
+// the output region's), rows / width of the output region; TO_LDS: into the
+// stored region `out`, else into the in-grid cells of the feature map y
+// The layer's weights are staged through `wtmp` (LDS the layer does not
+// read and has not yet written) once per workgroup, and each wave takes
+// its operands from there.
+template<bool RELU, bool TO_LDS>
+__device__ __forceinline__ void cnn_ft_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                             int rows, int width, int oy, int ox, int Hq, int Wq,
+                                             const uint16_t* __restrict__ wl, const float* __restrict__ bias,
+                                             uint16_t* __restrict__ y, uint4* __restrict__ wtmp) {
+    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5, wave = threadIdx.x >> 6;
+    constexpr int WQ = CNN_C * 9 * CNN_C / 8;  // 16-byte chunks of the layer's weights
+    for (int i = threadIdx.x; i < WQ; i += CNN_FT_THREADS) wtmp[i] = ((const uint4*)wl)[i];
+    __syncthreads();
+    bf16x8 wa[18];
+#pragma unroll
+    for (int s = 0; s < 18; ++s)
+        wa[s] = __builtin_bit_cast(bf16x8, wtmp[((nl * 9 + (s >> 1)) * CNN_C + 16 * (s & 1) + 8 * g) / 8]);
+    __syncthreads();
+    float bv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) bv[i] = bias[8 * (i / 4) + 4 * g + (i % 4)];
+    for (int r = wave; r < rows; r += CNN_FT_THREADS / 64) {
+        f32x16 acc = {0};
+#pragma unroll
+        for (int s = 0; s < 18; ++s) {
+            const int tap = s >> 1, ky = tap / 3, kx = tap % 3;
+            const bf16x8 bx = *(const bf16x8*)(in + ((r + ky) * CNN_FT_COLS + nl + kx) * CNN_FT_STRIDE + 16 * (s & 1) + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[s], bx, acc, 0, 0, 0);
+        }
+        const int Y = oy + r, X = ox + nl;
+        const bool inside = Y >= 0 && Y < Hq && X >= 0 && X < Wq;
+        uint32_t pk[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float o[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float v = acc[4 * q + k] + bv[4 * q + k];
+                o[k] = inside ? (RELU ? fmaxf(v, 0.0f) : v) : 0.0f;
+            }
+            pk[2 * q] = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
+            pk[2 * q + 1] = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
+        }
+        if constexpr (TO_LDS) {
+            if (nl < width) {
+                uint16_t* dst = out + (r * CNN_FT_COLS + nl) * CNN_FT_STRIDE + 4 * g;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) *(uint2*)(dst + 8 * q) = make_uint2(pk[2 * q], pk[2 * q + 1]);
+            }
+        } else {
+            if (nl < width && inside) {
+                uint16_t* dst = y + ((size_t)Y * Wq + X) * CNN_C + 4 * g;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) *(uint2*)(dst + 8 * q) = make_uint2(pk[2 * q], pk[2 * q + 1]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(CNN_FT_THREADS) void k_cnn_features_fused(
+    const uint16_t* __restrict__ xa, const uint16_t* __restrict__ xb, const uint16_t* __restrict__ w0,
+    const uint16_t* __restrict__ wm, const float* __restrict__ params, int Hq, int Wq,
+    uint16_t* __restrict__ ya, uint16_t* __restrict__ yb) {
+    constexpr int XR = CNN_FT_H + 8, A_ROWS = CNN_FT_H + 6, B_ROWS = CNN_FT_H + 4;
+    __shared__ uint2 xin[XR * CNN_FT_COLS];  // input cells (4 channels), rows Y0-4.., cols X0-4..
+    __shared__ uint4 bufA_raw[A_ROWS * CNN_FT_COLS * CNN_FT_STRIDE / 8];
+    __shared__ uint4 bufB_raw[B_ROWS * CNN_FT_COLS * CNN_FT_STRIDE / 8];
+    static_assert(B_ROWS * CNN_FT_COLS * CNN_FT_STRIDE >= CNN_C * 9 * CNN_C, "a layer's weights fit either buffer");
+    uint16_t* bufA = (uint16_t*)bufA_raw;
+    uint16_t* bufB = (uint16_t*)bufB_raw;
+    const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5, wave = threadIdx.x >> 6;
+    const int tiles_x = (Wq + CNN_FT_W - 1) / CNN_FT_W;
+    const int X0 = (blockIdx.x % tiles_x) * CNN_FT_W, Y0 = (blockIdx.x / tiles_x) * CNN_FT_H;
+    const uint16_t* x = blockIdx.z ? xb : xa;
+    uint16_t* y = blockIdx.z ? yb : ya;
+    for (int i = threadIdx.x; i < XR * CNN_FT_COLS; i += CNN_FT_THREADS) {
+        const int yy = Y0 - 4 + i / CNN_FT_COLS, xx = X0 - 4 + i % CNN_FT_COLS;
+        uint2 v = make_uint2(0, 0);
+        if (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq)
+            v = *(const uint2*)(x + ((size_t)yy * Wq + xx) * CNN_IN);
+        xin[i] = v;
+    }
+    __syncthreads();
+    {   // layer 0 (4 -> 32, ReLU) into A: rows Y0-3.., cols X0-3.. (k_cnn_conv0_mfma's products)
+        bf16x8 wa[CNN_K0 / 16];
+#pragma unroll
+        for (int st = 0; st < CNN_K0 / 16; ++st)
+            wa[st] = __builtin_bit_cast(bf16x8, *(const uint4*)(w0 + nl * CNN_K0 + 16 * st + 8 * g));
+        const float* bias = params + cnn_b_off(0);
+        for (int r = wave; r < A_ROWS; r += CNN_FT_THREADS / 64) {
+            f32x16 acc = {0};
+#pragma unroll
+            for (int st = 0; st < CNN_K0 / 16; ++st) {
+                const int t0 = 4 * st + 2 * g, t1 = t0 + 1;  // 4 channels of each tap's cell
+                uint2 lo = make_uint2(0, 0), hi = make_uint2(0, 0);
+                if (t0 < 9) lo = xin[(r + t0 / 3) * CNN_FT_COLS + nl + t0 % 3];
+                if (t1 < 9) hi = xin[(r + t1 / 3) * CNN_FT_COLS + nl + t1 % 3];
+                const bf16x8 bx = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wa[st], bx, acc, 0, 0, 0);
+            }
+            const int Y = Y0 - 3 + r, X = X0 - 3 + nl;
+            const bool inside = Y >= 0 && Y < Hq && X >= 0 && X < Wq;
+            uint16_t* dst = bufA + (r * CNN_FT_COLS + nl) * CNN_FT_STRIDE + 4 * g;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float o[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    o[k] = inside ? fmaxf(acc[4 * q + k] + bias[8 * q + 4 * g + k], 0.0f) : 0.0f;
+                *(uint2*)(dst + 8 * q) = make_uint2((uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16),
+                                                    (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
+            }
+        }
+    }
+    __syncthreads();
+    const size_t wl = (size_t)CNN_C * 9 * CNN_C;
+    // weights staged in the buffer each layer writes (B, A) or no longer
+    // needs (B, after a3)
+    cnn_ft_layer<true, true>(bufA, bufB, B_ROWS, CNN_FT_W + 4, Y0 - 2, X0 - 2, Hq, Wq, wm, params + cnn_b_off(1),
+                             nullptr, bufB_raw);
+    __syncthreads();
+    cnn_ft_layer<true, true>(bufB, bufA, CNN_FT_H + 2, CNN_FT_W + 2, Y0 - 1, X0 - 1, Hq, Wq, wm + wl,
+                             params + cnn_b_off(2), nullptr, bufA_raw);
+    __syncthreads();
+    cnn_ft_layer<false, false>(bufA, nullptr, CNN_FT_H, CNN_FT_W, Y0, X0, Hq, Wq, wm + 2 * wl,
+                               params + cnn_b_off(3), y, bufB_raw);
 }
 
 // Correlation and window of every cell of a 32-cell tile (one wave): the

@@ -163,7 +163,7 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
                 const float* params, const bicos_window_options* o, void* scratch, float* center,
                 int16_t* halfwidth, float* peak, hipStream_t s) {
     const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
-    if (g_cnn_impl == BICOS_CNN_MFMA) {
+    if (g_cnn_impl == BICOS_CNN_MFMA || g_cnn_impl == BICOS_CNN_MFMA_LAYERS) {
         if (g.K > CNN_KMAX_MFMA)
             return fail(BICOS_ERR_INVALID_ARGUMENT, "window bins above 160 need the vector-ALU network");
         uint8_t* p = (uint8_t*)scratch;
@@ -191,17 +191,24 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
         }
 #undef CI
         const int tiles_row = (g.Wq + 31) / 32;
+        if (g_cnn_impl == BICOS_CNN_MFMA) {
+            const int ft = ((g.Wq + CNN_FT_W - 1) / CNN_FT_W) * ((g.Hq + CNN_FT_H - 1) / CNN_FT_H);
+            LAUNCH("cnn_features", k_cnn_features_fused, dim3(ft, 1, 2), dim3(CNN_FT_THREADS), 0, s,
+                   (const uint16_t*)x0L, (const uint16_t*)x0R, (const uint16_t*)w0, (const uint16_t*)wm,
+                   params, g.Hq, g.Wq, bL, bR);
+        } else {
         LAUNCH("cnn_conv0", k_cnn_conv0_mfma, dim3((g.Hq * tiles_row + 3) / 4, 1, 2), block, 0, s,
                (const uint16_t*)x0L, (const uint16_t*)x0R, (const uint16_t*)w0, params + cnn_b_off(0),
                g.Hq, g.Wq, aL, aR);
         const dim3 cgrid(tiles_row * ((g.Hq + CNN_TROWS - 1) / CNN_TROWS), 1, 2);
         const size_t wl = (size_t)CNN_C * 9 * CNN_C;
-        LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, block, 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
+        LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, dim3(CNN_CONV_THREADS), 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
                (const uint16_t*)wm, params + cnn_b_off(1), g.Hq, g.Wq, bL, bR);
-        LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, block, 0, s, (const uint16_t*)bL, (const uint16_t*)bR,
+        LAUNCH("cnn_conv", k_cnn_conv_mfma<true>, cgrid, dim3(CNN_CONV_THREADS), 0, s, (const uint16_t*)bL, (const uint16_t*)bR,
                (const uint16_t*)(wm + wl), params + cnn_b_off(2), g.Hq, g.Wq, aL, aR);
-        LAUNCH("cnn_conv", k_cnn_conv_mfma<false>, cgrid, block, 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
+        LAUNCH("cnn_conv", k_cnn_conv_mfma<false>, cgrid, dim3(CNN_CONV_THREADS), 0, s, (const uint16_t*)aL, (const uint16_t*)aR,
                (const uint16_t*)(wm + 2 * wl), params + cnn_b_off(3), g.Hq, g.Wq, bL, bR);
+        }
         const int cw = g.Hq * tiles_row;
         const size_t clds = cnn_corr_window_lds(g.K);
         if (clds > 65536)  // above the default limit of dynamic LDS per workgroup
@@ -372,7 +379,7 @@ extern "C" {
 int bicos_hip_window_param_count(void) { return CNN_PARAMS; }
 
 int bicos_hip_set_cnn_impl(int32_t impl) {
-    if (impl != BICOS_CNN_MFMA && impl != BICOS_CNN_VALU)
+    if (impl != BICOS_CNN_MFMA && impl != BICOS_CNN_VALU && impl != BICOS_CNN_MFMA_LAYERS)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown network implementation");
     g_cnn_impl = impl;
     return BICOS_OK;

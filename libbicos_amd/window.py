@@ -63,12 +63,13 @@ def _check_params(params):
         raise Exception(f"window parameters must be a contiguous float32 tensor of {PARAM_COUNT}")
 
 
-CNN_MFMA, CNN_VALU = 0, 1
+CNN_MFMA, CNN_VALU, CNN_MFMA_LAYERS = 0, 1, 2
 
 
 def set_cnn_impl(impl: int) -> None:
     """Network implementation of predict / match_windowed_device for later
-    calls: CNN_MFMA (default, matrix cores) or CNN_VALU (training kernels)."""
+    calls: CNN_MFMA (default, matrix cores, layers fused), CNN_MFMA_LAYERS
+    (matrix cores, one kernel per layer) or CNN_VALU (training kernels)."""
     _check(_lib.bicos_hip_set_cnn_impl(int(impl)))
 
 

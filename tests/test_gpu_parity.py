@@ -70,6 +70,18 @@ def test_transform_bit_exact(gpu, oracle, dtype, full):
             assert np.array_equal(got, exp), (n, levels)
 
 
+def test_transform_deep_u16_aligned(gpu, oracle):
+    """16-bit stacks deeper than 33 frames at row-aligned widths: the
+    2-pixel-per-lane limited kernel (256-bit descriptors)."""
+    rng = np.random.default_rng(7)
+    for n, cols in ((34, 1028), (48, 772), (64, 4096), (65, 516)):
+        for levels in (None, 3):
+            s = rand_stack(rng, n, 12, cols, np.uint16, levels)
+            exp = oracle.transform(s, full=False)
+            got = host(gpu.transform(dev(s), 0), np.uint32)
+            assert np.array_equal(got, exp), (n, cols, levels)
+
+
 def test_transform_wider_than_needed(gpu, oracle):
     rng = np.random.default_rng(3)
     s = rand_stack(rng, 9, 16, 300, np.uint16)

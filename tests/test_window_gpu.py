@@ -211,6 +211,26 @@ def test_mfma_network_matches_valu(W, bins):
     assert np.median(np.abs(pm - pv)) <= 1e-3
 
 
+@pytest.mark.parametrize("bins", [80, 144])
+def test_fused_network_equals_layers(W, bins):
+    """The four layers in one kernel give the per-layer kernels' windows
+    bit for bit (same products and rounding; the crop's 25 x 162 cells leave
+    partial tiles at the right and bottom edges)."""
+    from libbicos_amd import scene
+    L, R, _ = W.scene_device(scene.C1_1920x1200_U16, 78, frames=4, height=100, width=648)
+    p = W.load()[0] if os.path.exists(W.DEFAULT_WEIGHTS) else W.init_params(6)
+    params = W.params_device(p)
+    opts = W.WindowOptions(bins=bins)
+    try:
+        W.set_cnn_impl(W.CNN_MFMA_LAYERS)
+        ref = [host(t) for t in W.predict(L, R, params, opts)]
+    finally:
+        W.set_cnn_impl(W.CNN_MFMA)
+    got = [host(t) for t in W.predict(L, R, params, opts)]
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b, equal_nan=True)
+
+
 # ------------------------------------------------------ windowed matching
 
 def c1_crop(W, seed, H=64, Wd=480, frames=33):
