@@ -261,10 +261,24 @@ def _img_of_tensor2d(t) -> _Image:
 
 
 def _stack_images(stack) -> "C.Array":
-    """stack: (n, rows, cols) tensor or a sequence of (rows, cols) tensors."""
-    frames = [stack[i] for i in range(stack.shape[0])] if hasattr(stack, "dim") else list(stack)
+    """stack: (n, rows, cols) tensor or a sequence of (rows, cols) tensors.
+    The array keeps the tensors alive (`_keep`) and records their device."""
+    if hasattr(stack, "dim"):
+        if stack.dim() != 3 or stack.stride(2) != 1:
+            raise Exception("images must be 2-D with unit column stride")
+        # frame descriptors straight from the base pointer and strides (no
+        # per-frame tensor views: this runs on every call)
+        n, rows, cols = stack.shape
+        es = stack.element_size()
+        base, fstride, step, code = stack.data_ptr(), stack.stride(0) * es, stack.stride(1) * es, _tdtype_code(stack)
+        arr = (_Image * n)(*[_Image(base + i * fstride, rows, cols, code, 0, step) for i in range(n)])
+        arr._keep = (stack,)
+        arr._device = stack.device
+        return arr
+    frames = list(stack)
     arr = (_Image * len(frames))(*[_img_of_tensor2d(f) for f in frames])
     arr._keep = frames
+    arr._device = frames[0].device if frames else None
     return arr
 
 
@@ -288,7 +302,7 @@ def match_device(stack0, stack1, cfg: Optional[Config] = None, disparity=None, c
     if len(s1) != n:
         raise Exception("both stacks need the same number of images")
     rows, cols = s0[0].rows, s0[0].cols
-    dev = s0._keep[0].device
+    dev = s0._device
     dt = torch.float32 if _lib.bicos_hip_disparity_type(C.byref(c)) == T_32F else torch.int16
     if disparity is None or disparity.dtype != dt or tuple(disparity.shape) != (rows, cols):
         disparity = torch.empty((rows, cols), dtype=dt, device=dev)
@@ -319,7 +333,7 @@ def transform(stack, mode: int, words: Optional[int] = None, out=None, stream=No
             raise ValueError("input stacks too large")
         words = bits // 32
     if out is None:
-        out = torch.empty((rows, cols, words), dtype=torch.int32, device=imgs._keep[0].device)
+        out = torch.empty((rows, cols, words), dtype=torch.int32, device=imgs._device)
     _check(_lib.bicos_hip_transform(imgs, n, mode, words, C.c_void_p(out.data_ptr()),
                                     _stream_ptr(stream)))
     return out

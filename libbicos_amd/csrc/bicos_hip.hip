@@ -63,6 +63,7 @@ struct Workspace {
     void* buf = nullptr;
     size_t cap = 0;
     hipEvent_t last = nullptr;
+    hipStream_t last_stream = nullptr;  // stream of the last use
     bool used = false;
 };
 
@@ -82,7 +83,9 @@ int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out, int slot) {
         w.cap = want;
         w.used = false;
     }
-    if (w.used) HIP_TRY(hipStreamWaitEvent(s, w.last, 0));
+    // the last use on another stream must finish first (on the same stream,
+    // stream order already guarantees it)
+    if (w.used && w.last_stream != s) HIP_TRY(hipStreamWaitEvent(s, w.last, 0));
     *out = w.buf;
     return BICOS_OK;
 }
@@ -91,6 +94,7 @@ int ws_release(int dev, hipStream_t s, int slot) {
     Workspace& w = g_ws[dev * WS_SLOTS + slot];
     HIP_TRY(hipEventRecord(w.last, s));
     w.used = true;
+    w.last_stream = s;
     return BICOS_OK;
 }
 

@@ -118,7 +118,7 @@ def predict(stack0, stack1, params, opts: Optional[WindowOptions] = None, stream
     s0, s1 = _stack_images(stack0), _stack_images(stack1)
     n = len(s0)
     Hq, Wq = s0[0].rows // CELL, s0[0].cols // CELL
-    dev = s0._keep[0].device
+    dev = s0._device
     center = torch.empty((Hq, Wq), dtype=torch.float32, device=dev)
     hw = torch.empty((Hq, Wq), dtype=torch.int16, device=dev)
     peak = torch.empty((Hq, Wq), dtype=torch.float32, device=dev)
@@ -171,7 +171,7 @@ def match_windowed_device(stack0, stack1, cfg: Optional[Config], params,
     if len(s1) != n:
         raise Exception("both stacks need the same number of images")
     rows, cols = s0[0].rows, s0[0].cols
-    dev = s0._keep[0].device
+    dev = s0._device
     dt = torch.float32 if _lib.bicos_hip_disparity_type(C.byref(c)) == T_32F else torch.int16
     if disparity is None or disparity.dtype != dt or tuple(disparity.shape) != (rows, cols):
         disparity = torch.empty((rows, cols), dtype=dt, device=dev)
