@@ -97,6 +97,34 @@ def cpu_baseline(L, R, cfg_kw):
                       f"{threads} threads, oracle/bicos_oracle.cpp)"}
 
 
+def pointcloud_timing(B, disp, args):
+    """Point cloud of the plain path's disparity (the reference CLI's --q-store
+    path): reprojection with the Q of a rectified rig, then the selection
+    save_pointcloud applies; the selection returns its counts, so each call
+    ends with a stream synchronisation."""
+    import torch
+    H, W = disp.shape
+    f, B_m = 1400.0, 0.1
+    Q = [[1, 0, 0, -(W - 1) / 2], [0, 1, 0, -(H - 1) / 2], [0, 0, 0, f], [0, 0, 1 / B_m, 0]]
+    B.reproject(disp, Q)
+    B.pointcloud(disp, Q)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        B.reproject(disp, Q)
+    e1.record()
+    torch.cuda.synchronize()
+    rep_ms = e0.elapsed_time(e1) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, counts = B.pointcloud(disp, Q)
+    pc_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    return {"reproject_ms": round(rep_ms, 4), "reproject_plus_select_ms": round(pc_ms, 4),
+            "points": counts["written"],
+            "reproject_GBps": round(W * H * (4 + 12) / (rep_ms * 1e-3) / 1e9, 1)}
+
+
 def learned_path(B, dL, dR, plain_disp, args, W, H, plain_ms):
     """The learned-window path (MOMENTS subpixel stage, committed weights)
     on the same resident pair: timed like the plain path, then compared with
@@ -274,6 +302,7 @@ def main():
     d = disp.cpu().numpy()
     valid = float(np.isfinite(d).mean())
     learned = learned_path(B, dL, dR, disp, args, W, H, ms_per_step)
+    points = pointcloud_timing(B, disp, args)
 
     out = {
         "metric": METRIC,
@@ -296,6 +325,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "learned": learned,
+        "pointcloud": points,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(L, R, dict(thr=0.96, step=0.1, min_variance=2.0,

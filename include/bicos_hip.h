@@ -278,6 +278,23 @@ int bicos_hip_scene_generate(int32_t width, int32_t height, int32_t n, int32_t d
                              double p_dark, double dark_value, void* left, void* right,
                              float* disp, void* stream);
 
+/* ---- point clouds (the reference CLI's --q-store path, src/cli.cpp:228-250) ---- */
+
+/* cv::reprojectImageTo3D(disparity, points, Q, false, CV_32F) as
+ * src/cli.cpp:239 calls it: for the pixel in column x, row y with disparity d,
+ * [X Y Z W] = Q [x y d 1] in double precision, point = (X/W, Y/W, Z/W) as
+ * float. disparity: device BICOS_16S (whole pixels) or BICOS_32F image; Q:
+ * host, 16 doubles row-major; points: device rows*cols*3 floats. */
+int bicos_hip_reproject(const bicos_image* disparity, const double* Q, float* points, void* stream);
+
+/* The selection of save_pointcloud (include/fileutils.hpp:45-90): points of
+ * valid disparities with finite coordinates and, unless allow_negative_z,
+ * z >= 0, packed in row-major order into xyz (device, room for rows*cols*3
+ * floats). counts (host, 3 values): points written, skipped as non-finite,
+ * skipped for negative z. Waits for the stream (the counts are returned). */
+int bicos_hip_pointcloud(const bicos_image* disparity, const float* points, int32_t allow_negative_z,
+                         float* xyz, int64_t* counts, void* stream);
+
 /* ---- measurement ---- */
 
 /* Per-kernel GPU time accounting with HIP events recorded around every

@@ -112,6 +112,8 @@ def _load():
     d = C.c_double
     L.bicos_hip_scene_generate.argtypes = [i32, i32, i32, i32, d, d, C.c_uint64, d, d, d, d, d,
                                            vp, vp, vp, vp]
+    L.bicos_hip_reproject.argtypes = [P(_Image), P(C.c_double), vp, vp]
+    L.bicos_hip_pointcloud.argtypes = [P(_Image), vp, i32, vp, P(C.c_int64), vp]
     return L
 
 
@@ -127,6 +129,7 @@ EXPORTED_SYMBOLS = (
     "bicos_hip_window_options_default", "bicos_hip_window_init", "bicos_hip_window_label",
     "bicos_hip_window_loss_grad", "bicos_hip_window_adam", "bicos_hip_window_predict",
     "bicos_hip_match_windowed", "bicos_hip_scene_generate", "bicos_hip_set_cnn_impl",
+    "bicos_hip_reproject", "bicos_hip_pointcloud",
 )
 
 
@@ -319,6 +322,36 @@ def match_device(stack0, stack1, cfg: Optional[Config] = None, disparity=None, c
     _check(_lib.bicos_hip_match(s0, s1, n, C.byref(c), C.byref(dimg),
                                 C.byref(cimg) if cimg is not None else None, _stream_ptr(stream)))
     return disparity, corrmap
+
+
+def reproject(disparity, Q, stream=None):
+    """cv::reprojectImageTo3D(disparity, points, Q, false, CV_32F) as the
+    reference CLI calls it (src/cli.cpp:239): (rows, cols, 3) float32 points
+    of an int16 or float32 CUDA disparity tensor; Q: 4x4 (row-major)."""
+    torch = _torch()
+    q = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).reshape(16))
+    img = _img_of_tensor2d(disparity)
+    pts = torch.empty((disparity.shape[0], disparity.shape[1], 3), dtype=torch.float32,
+                      device=disparity.device)
+    _check(_lib.bicos_hip_reproject(C.byref(img), q.ctypes.data_as(C.POINTER(C.c_double)),
+                                    C.c_void_p(pts.data_ptr()), _stream_ptr(stream)))
+    return pts
+
+
+def pointcloud(disparity, Q, allow_negative_z: bool = False, stream=None):
+    """The points save_pointcloud writes (include/fileutils.hpp:45-90): those of
+    valid disparities with finite coordinates (and z >= 0 unless allowed), in
+    row-major order. Returns ((N, 3) float32 CUDA tensor, {"written",
+    "nonfinite", "negative_z"})."""
+    torch = _torch()
+    pts = reproject(disparity, Q, stream)
+    img = _img_of_tensor2d(disparity)
+    xyz = torch.empty((disparity.numel(), 3), dtype=torch.float32, device=disparity.device)
+    counts = (C.c_int64 * 3)()
+    _check(_lib.bicos_hip_pointcloud(C.byref(img), C.c_void_p(pts.data_ptr()), int(bool(allow_negative_z)),
+                                     C.c_void_p(xyz.data_ptr()), counts, _stream_ptr(stream)))
+    n = int(counts[0])
+    return xyz[:n], {"written": n, "nonfinite": int(counts[1]), "negative_z": int(counts[2])}
 
 
 def transform(stack, mode: int, words: Optional[int] = None, out=None, stream=None):

@@ -71,7 +71,7 @@ int launch(const char* name, Kernel kernel, dim3 grid, dim3 block, size_t lds, h
 // g_ws_mu. ws_acquire returns a buffer of at least `bytes` that is free for
 // use on `s` (the stream waits for the last call that used it); ws_release
 // marks the end of that use on `s`.
-enum WorkspaceSlot : int { WS_MATCH = 0, WS_SCENE = 1, WS_WINDOW = 2, WS_SLOTS = 3 };
+enum WorkspaceSlot : int { WS_MATCH = 0, WS_SCENE = 1, WS_WINDOW = 2, WS_POINTS = 3, WS_SLOTS = 4 };
 extern std::mutex g_ws_mu;
 int ws_acquire(int dev, size_t bytes, hipStream_t s, void** out, int slot = WS_MATCH);
 int ws_release(int dev, hipStream_t s, int slot = WS_MATCH);

@@ -498,3 +498,37 @@ def test_precision_half_mfma_u8_full_mode(gpu, oracle):
     print("HALF_MFMA u8 vs FP64:", st)
     assert st["valid_both"] > 0.5 * got.size
     assert st["d_mean"] < 0.05 and st["flip_frac"] < 0.02
+
+
+# ------------------------------------------------------------ point clouds
+
+@pytest.mark.parametrize("kind", ["int16", "float32"])
+def test_pointcloud_matches_oracle(gpu, kind):
+    """bicos_hip_reproject / bicos_hip_pointcloud against oracle/pointcloud.py
+    (cv::reprojectImageTo3D as src/cli.cpp:239 calls it, then save_pointcloud's
+    selection): points bit for bit, the same selection and counts. The Q of a
+    rectified rig with a principal-point offset makes W = 0 (non-finite
+    points) at d = 40 and z < 0 below it."""
+    import pointcloud as PC
+    rng = np.random.default_rng(21 + (kind == "float32"))
+    rows, cols = 37, 301
+    if kind == "int16":
+        d = rng.integers(-60, 300, size=(rows, cols)).astype(np.int16)
+        d[rng.random((rows, cols)) < 0.1] = -32768
+        d[rng.random((rows, cols)) < 0.03] = 40
+    else:
+        d = rng.uniform(-60.0, 300.0, size=(rows, cols)).astype(np.float32)
+        d[rng.random((rows, cols)) < 0.1] = np.nan
+        d[rng.random((rows, cols)) < 0.03] = 40.0
+    f, cx, cy, B = 812.5, 150.25, 18.5, 0.095
+    q32 = 1 / B
+    Q = [[1, 0, 0, -cx], [0, 1, 0, -cy], [0, 0, 0, f], [0, 0, q32, -(q32 * 40.0)]]   # W = 0 at d = 40
+    exp_pts = PC.reproject(d, Q)
+    got_pts = host(gpu.reproject(dev(d), Q))
+    assert np.array_equal(got_pts, exp_pts, equal_nan=True)
+    for allow in (False, True):
+        xyz, c = gpu.pointcloud(dev(d), Q, allow_negative_z=allow)
+        exp, n, nonfin, negz = PC.select(d, exp_pts, allow)
+        assert (c["written"], c["nonfinite"], c["negative_z"]) == (n, nonfin, negz)
+        assert nonfin > 0 and (allow or negz > 0)
+        assert np.array_equal(host(xyz), exp)

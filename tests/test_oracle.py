@@ -319,3 +319,52 @@ def test_match_dispatch_and_errors(oracle):
     assert oracle.required_bits(33, False) == 125 and oracle.descriptor_words(33, False) == 4
     assert oracle.required_bits(64, False) == 249 and oracle.descriptor_words(64, False) == 8
     assert oracle.required_bits(12, True) == 123
+
+
+# --------------------------------------------------------- point clouds
+# oracle/pointcloud.py restates cv::reprojectImageTo3D (src/cli.cpp:239) and
+# save_pointcloud (include/fileutils.hpp:45-90); OpenCV is absent, so the
+# restatement is pinned by closed forms.
+
+def test_reproject_rectified_rig_closed_form():
+    """Q of a rectified pair (focal f, principal point (cx, cy), baseline B,
+    Tx = -B): the point of disparity d at (x, y) is ((x - cx), (y - cy), f) * B / d."""
+    import pointcloud as PC
+    f, cx, cy, B = 800.0, 319.5, 239.25, 0.12
+    Q = [[1, 0, 0, -cx], [0, 1, 0, -cy], [0, 0, 0, f], [0, 0, 1 / B, 0]]
+    rng = np.random.default_rng(11)
+    d = rng.integers(1, 200, size=(6, 9)).astype(np.int16)
+    p = PC.reproject(d, Q).astype(np.float64)
+    y, x = np.mgrid[0:6, 0:9]
+    s = B / d.astype(np.float64)
+    exp = np.stack([(x - cx) * s, (y - cy) * s, f * s], axis=-1)
+    assert np.allclose(p, exp, rtol=1e-6, atol=1e-6)
+    # a flipped sign, a swapped x/y or a dropped translation fails this
+    assert np.all(p[..., 2] > 0)
+
+
+def test_reproject_identity_and_float_disparity():
+    import pointcloud as PC
+    d = np.array([[0.5, -3.25], [np.nan, 7.0]], np.float32)
+    p = PC.reproject(d, np.eye(4))
+    assert p[0, 1].tolist() == [1.0, 0.0, -3.25] and p[1, 1].tolist() == [1.0, 1.0, 7.0]
+    assert np.isnan(p[1, 0]).all()                                   # 0 * NaN is NaN
+
+
+def test_pointcloud_selection_hand_counted():
+    """invalid disparities skipped silently; W = 0 gives non-finite points
+    (counted); z < 0 counted unless allowed; row-major order."""
+    import pointcloud as PC
+    Q = [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 1, 0]]   # W = d
+    d = np.array([[2, -32768, 0], [-4, 5, 1]], np.int16)            # W = 0 at (0, 2)
+    p = PC.reproject(d, Q)
+    pts, n, nonfin, negz = PC.select(d, p)
+    assert (n, nonfin, negz) == (4, 1, 0)                            # z = d / d = 1 wherever finite
+    assert pts.tolist() == [[0.0, 0.0, 1.0], [0.0, -0.25, 1.0], [0.2, 0.2, 1.0], [2.0, 1.0, 1.0]] or \
+        np.allclose(pts, [[0, 0, 1], [0, -0.25, 1], [0.2, 0.2, 1], [2, 1, 1]], rtol=1e-7)
+    Qn = [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, -1, 0], [0, 0, 0, 1]]  # z = -d, W = 1
+    pts, n, nonfin, negz = PC.select(d, PC.reproject(d, Qn))
+    assert (n, nonfin, negz) == (2, 0, 3)                            # z = -0.0 is not negative
+    assert pts[:, 2].tolist() == [0.0, 4.0]
+    pts, n, nonfin, negz = PC.select(d, PC.reproject(d, Qn), allow_negative_z=True)
+    assert (n, nonfin, negz) == (5, 0, 0)
