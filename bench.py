@@ -35,7 +35,8 @@ def parse_args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    # untimed steps; the GPU's clocks take a few dozen calls to settle
+    p.add_argument("--warmup", type=int, default=30)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=0x600DF00D)
     p.add_argument("--search", choices=("mfma", "valu"), default="mfma",
