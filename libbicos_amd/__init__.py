@@ -89,6 +89,7 @@ def _load():
     L.bicos_hip_agree_subpixel.argtypes = [vp, P(_Image), P(_Image), u32, f32, f32, i32, f32,
                                            i32, vp, vp, vp]
     L.bicos_hip_set_search_impl.argtypes = [i32]
+    L.bicos_hip_set_subpixel_impl.argtypes = [i32]
     L.bicos_hip_timing_enable.argtypes = [i32]
     L.bicos_hip_timing_read.argtypes = [P(_KernelStat), i32]
     L.bicos_hip_malloc.argtypes = [C.c_size_t]
@@ -122,7 +123,8 @@ _lib = _load()
 EXPORTED_SYMBOLS = (
     "bicos_hip_config_default", "bicos_hip_disparity_type", "bicos_hip_corrmap_type",
     "bicos_hip_descriptor_bits", "bicos_hip_match", "bicos_hip_last_error",
-    "bicos_hip_release_workspace", "bicos_hip_set_search_impl", "bicos_hip_transform", "bicos_hip_search", "bicos_hip_agree",
+    "bicos_hip_release_workspace", "bicos_hip_set_search_impl", "bicos_hip_set_subpixel_impl",
+    "bicos_hip_transform", "bicos_hip_search", "bicos_hip_agree",
     "bicos_hip_agree_subpixel", "bicos_hip_timing_enable", "bicos_hip_timing_read",
     "bicos_hip_malloc", "bicos_hip_free", "bicos_hip_memcpy_h2d", "bicos_hip_memcpy_d2h",
     "bicos_hip_device_synchronize", "bicos_hip_window_param_count",
@@ -429,6 +431,16 @@ SEARCH_MFMA, SEARCH_VALU = 0, 1
 def set_search_impl(impl: int) -> None:
     """Row-search kernel for later calls: SEARCH_MFMA (default) or SEARCH_VALU."""
     _check(_lib.bicos_hip_set_search_impl(int(impl)))
+
+
+SUBPIXEL_PRUNED, SUBPIXEL_SCAN = 0, 1
+
+
+def set_subpixel_impl(impl: int) -> None:
+    """Subpixel kernel at Precision.SINGLE for later calls: SUBPIXEL_PRUNED
+    (default; closed-form bounds pick the grid points the exact arithmetic
+    scores) or SUBPIXEL_SCAN (every grid point). Identical results."""
+    _check(_lib.bicos_hip_set_subpixel_impl(int(impl)))
 
 
 def release_workspace() -> None:

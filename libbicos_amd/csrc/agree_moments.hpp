@@ -23,10 +23,15 @@
 // two-pass sums need them), not held in registers.
 #pragma once
 
+#include <type_traits>
+
 #include "agree.hpp"
 #include "device_common.hpp"
 
 namespace bicos_hip {
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // The reference's nxcorr (agree.hpp:27-50) on the left pixel (col) and the
 // right samples it(t): means, then fma chains in frame order, streaming the
@@ -206,13 +211,186 @@ __device__ void exact_pending_wave(uint64_t pend, int row, int col, int d, const
     }
 }
 
+// The grid points x_k of the subpixel scan (x_0 = -1, x_k = x_(k-1) + step
+// in FP32, agree.hpp:160), formed on the host the way the reference's loop
+// forms them; used where npts <= 64.
+struct GridX {
+    float x[64];
+};
+
+// Pixels the exact candidate stage takes per round of one wave (each brings
+// its candidate grid points, one lane per point, 64 lanes per round); fewer
+// for deep stacks, so that a workgroup's staging stays near 32 KiB of LDS.
+template<int NMAX>
+struct CandLds {
+    static constexpr int CAND_PIXELS = NMAX <= 36 ? 16 : 8;
+    float4 xs[CAND_PIXELS * NMAX];  // (left sample - mean0, a, b, c) of frame t
+    int row[CAND_PIXELS], col[CAND_PIXELS], c1[CAND_PIXELS], pre[CAND_PIXELS];
+    float mean0[CAND_PIXELS];
+    uint64_t mask[CAND_PIXELS];
+    int start[64];
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    // the wave's LDS operations complete in order: a fence keeps the compiler
+    // from moving LDS accesses across this point
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The exact subpixel stage (agree.hpp:94-185 arithmetic, bit-identical to
+// k_agree_subpixel) restricted to candidate grid points: the lanes with
+// `need` set hold in `cmask` (bit k: grid point k) a set of points that
+// contains every point whose exact correlation can be the scan's maximum
+// (every other point is strictly below some point of the set, so the
+// reference's first strict maximum lies in the set and is found by scanning
+// the set in grid order). Lanes are handed out one per (pixel, candidate)
+// pair: per round, the pending pixels in lane order up to CAND_PIXELS pixels
+// or 64 pairs. A round stages each pixel's (left sample - mean0, a, b, c) for
+// all frames in LDS, every pair lane runs both of the reference's passes on
+// its point there, and each pixel's lanes reduce to the first strict maximum
+// (larger correlation, then smaller k). Sets res as the reference would and
+// clears need. Must be reached by all 64 lanes; npts <= 64.
+//
+// inrange: every frame's parabola of the lane's pixel stays inside
+// [2, max(TIn) - 2], so the cast of agree.hpp:164 is the identity; a round
+// whose pixels all have it skips the cast.
+template<typename TIn, int NMAX, int N_EXACT>
+__device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int col, int d,
+                                      float mean0_own, bool inrange, const Stack& f0,
+                                      const Stack& f1, Depth<NMAX, N_EXACT> n, float thr,
+                                      const GridX& gx, bool mv, float minvar, float& res,
+                                      CandLds<NMAX>& L) {
+    constexpr int CAND_PIXELS = CandLds<NMAX>::CAND_PIXELS;
+    const int lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int nf = n.get();
+    const int cnt = need ? __popcll(cmask) : 0;  // 1..64 where need
+    while (__ballot(need)) {
+        // exclusive prefix of the pair counts over the pending lanes
+        const int c = need ? cnt : 0;
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const int pre = incl - c;
+        const int rank = __popcll(__ballot(need) & lt);
+        const bool take = need && incl <= 64 && rank < CAND_PIXELS;  // a prefix of the pending lanes
+        const uint64_t takers = __ballot(take);
+        const int last = 63 - __builtin_clzll(takers);
+        const int total = __builtin_amdgcn_readlane(incl, last);
+        const int ntake = __popcll(takers);
+        L.start[lane] = 0;
+        wave_lds_sync();
+        if (take) {
+            L.row[rank] = row;
+            L.col[rank] = col;
+            L.c1[rank] = col - d;
+            L.pre[rank] = pre;
+            L.mean0[rank] = mean0_own;
+            L.mask[rank] = cmask;
+            L.start[pre] = 1;
+        }
+        wave_lds_sync();
+        constexpr int PER_LANE = (CAND_PIXELS * NMAX + 63) / 64;
+#pragma unroll 2
+        for (int q = 0; q < PER_LANE; ++q) {
+            const int e = q * 64 + lane;
+            const int gg = e / nf, t = e - gg * nf;
+            if (gg < ntake) {
+                const int r = L.row[gg], cc = L.col[gg], c1 = L.c1[gg];
+                const float u = (float)load_sample_cached<TIn>(f0, t, r, cc);
+                const float y0 = (float)load_sample_cached<TIn>(f1, t, r, c1 - 1);
+                const float y1 = (float)load_sample_cached<TIn>(f1, t, r, c1);
+                const float y2 = (float)load_sample_cached<TIn>(f1, t, r, c1 + 1);
+                L.xs[gg * NMAX + t] = make_float4(u - L.mean0[gg], 0.5f * (y0 - 2.0f * y1 + y2),
+                                                  0.5f * (-y0 + y2), y1);
+            }
+        }
+        // this lane's pair: pixel p (rank among the takers), its j-th candidate
+        const bool on = lane < total;
+        const uint64_t starts = __ballot(L.start[lane] != 0);
+        wave_lds_sync();
+        const int p = on ? __popcll(starts & (lt | (1ull << lane))) - 1 : 0;
+        const int p_pre = L.pre[p];
+        uint64_t m = L.mask[p];
+        const int seg_end = p_pre + __popcll(m);
+        for (int i = p_pre; i < lane; ++i) m &= m - 1ull;
+        const int k = on ? __builtin_ctzll(m) : 0;
+        const float xe = gx.x[k];
+        const float4* px = L.xs + p * NMAX;
+        const bool fast = __ballot(take && !inrange) == 0;  // uniform over the wave
+        // frame t's interpolated sample at xe (agree.hpp:164); c0 = left - mean0
+        const auto sample = [&](const float4& v) {
+            const float r = __builtin_rintf(v.y * xe * xe + v.z * xe + v.w);
+            return fast ? r : (float)(TIn)(int)r;
+        };
+        // both of the reference's passes recompute the samples (keeping them
+        // in registers would double the kernel's register count)
+        float mean1 = 0.0f, covar = 0.0f, var0 = 0.0f, var1 = 0.0f;
+#pragma unroll 8
+        for (int t = 0; t < NMAX; ++t)
+            if (n.has(t)) mean1 += sample(px[t]);
+        mean1 /= (float)nf;
+#pragma unroll 8
+        for (int t = 0; t < NMAX; ++t)
+            if (n.has(t)) {
+                const float4 v = px[t];
+                const float d1 = sample(v) - mean1;
+                var0 = __builtin_fmaf(v.x, v.x, var0);
+                covar = __builtin_fmaf(v.x, d1, covar);
+                var1 = __builtin_fmaf(d1, d1, var1);
+            }
+        float best = (mv && (var0 < minvar || var1 < minvar)) ? -1.0f : covar / psqrt(var0 * var1);
+        int best_k = k;
+        if (!on || !(best > -1.0f)) {  // does not beat the initial -1 (NaN included)
+            best = -1.0f;
+            best_k = 0x7FFFFFFF;
+        }
+        // segmented reduction towards each pixel's first lane
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float ob = __shfl_down(best, o);
+            const int ok = __shfl_down(best_k, o);
+            if (lane + o < seg_end && (ob > best || (ob == best && ok < best_k))) {
+                best = ob;
+                best_k = ok;
+            }
+        }
+        const float hb = __shfl(best, take ? pre : lane);
+        const int hk = __shfl(best_k, take ? pre : lane);
+        if (take) {
+            // the reference keeps x = 0 unless some point beats -1 (agree.hpp:159,168-171)
+            const float bx = hk != 0x7FFFFFFF ? gx.x[hk] : 0.0f;
+            res = hb < thr ? NAN : (float)d - bx;
+            need = false;
+        }
+        wave_lds_sync();  // the next round's staging must not overtake this round's reads
+    }
+}
+
 // act: the lane's pixel is real (row and col are valid either way; inactive
 // lanes compute but write nothing). Must be reached by all 64 lanes.
-template<typename TIn, int NMAX, int N_EXACT>
+//
+// BX = true: the bit-exact form (Precision::SINGLE without a corrmap,
+// npts <= 64). The moments are exact (FP64 sums of integer products), and a
+// settled pixel's candidate points are those whose closed form plus bound
+// reaches the largest closed form minus bound; any other point's exact
+// correlation is strictly below that point's. A single candidate that clears
+// the threshold (and -1) by its bound is the reference's result as is;
+// otherwise the exact candidate stage scores the candidates (all points for
+// an unsettled pixel). Output equals k_agree_subpixel's bit for bit; no
+// status is written. `lds` is the wave's CandLds (BX) or float4 slice.
+template<typename TIn, int NMAX, int N_EXACT, bool BX = false>
 __device__ __forceinline__ void agree_subpixel_moments_px(
     bool act, int row, int col, const int16_t* __restrict__ disp, int64_t dstep, const Stack& f0,
     const Stack& f1, int n_rt, int W, float thr, float step, int npts, int has_minvar, float minvar,
-    float* __restrict__ out, int64_t ostep, uint8_t* __restrict__ status, float4* __restrict__ xs) {
+    float* __restrict__ out, int64_t ostep, uint8_t* __restrict__ status, void* lds,
+    const GridX* gx = nullptr) {
+    using MT = typename std::conditional<BX, double, float>::type;  // cross-moment sums
     const Depth<NMAX, N_EXACT> n{n_rt};
     const bool mv = has_minvar != 0;
     const int d = disp[(size_t)row * dstep + col];
@@ -230,8 +408,9 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     // first frame's (products below 2^32; relative error < 1e-5 of the
     // centred moments, covered by the bound's slack below).
     double su = 0, suu = 0;
-    float s0 = 0, s1 = 0, s2 = 0, su0 = 0, su1 = 0, su2 = 0;
-    float s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0, su_f = 0;
+    MT s0 = 0, s1 = 0, s2 = 0, su0 = 0, su1 = 0, su2 = 0;
+    MT s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0, su_f = 0;
+    uint64_t cmask = 0;  // BX: candidate grid points of a pending pixel
     int ylo = 0x7FFFFFFF, yhi = -0x7FFFFFFF;
     // branch-free loop (the right samples of lanes without an interior match
     // are read at a valid column and ignored), so that loads of many frames
@@ -240,7 +419,9 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     const int y_ref0 = (int)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
     const int y_ref1 = (int)load_sample_cached<TIn>(f1, 0, row, c1);
     const int y_ref2 = (int)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
-#pragma unroll 11
+    // frames in flight per step; fewer for the exact FP64 sums (registers)
+    constexpr int MOM_UNROLL = BX ? 8 : 11;
+#pragma unroll MOM_UNROLL
     for (int t = 0; t < NMAX; ++t)
         if (n.has(t)) {
             const int ui = (int)load_sample_cached<TIn>(f0, t, row, col);
@@ -257,15 +438,15 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             const int r4 = (abs(i2 - i0) + 3) >> 2;
             ylo = min(ylo, min(min(i0, i2), i1 - r4));
             yhi = max(yhi, max(max(i0, i2), i1 + r4));
-            const float us = (float)(ui - u_ref), y0 = (float)(i0 - y_ref0), y1 = (float)(i1 - y_ref1),
-                        y2 = (float)(i2 - y_ref2);
+            const MT us = (MT)(ui - u_ref), y0 = (MT)(i0 - y_ref0), y1 = (MT)(i1 - y_ref1),
+                     y2 = (MT)(i2 - y_ref2);
             su_f += us;
             s0 += y0; s1 += y1; s2 += y2;
-            su0 = __builtin_fmaf(us, y0, su0); su1 = __builtin_fmaf(us, y1, su1);
-            su2 = __builtin_fmaf(us, y2, su2);
-            s00 = __builtin_fmaf(y0, y0, s00); s01 = __builtin_fmaf(y0, y1, s01);
-            s02 = __builtin_fmaf(y0, y2, s02); s11 = __builtin_fmaf(y1, y1, s11);
-            s12 = __builtin_fmaf(y1, y2, s12); s22 = __builtin_fmaf(y2, y2, s22);
+            su0 = fma_t(us, y0, su0); su1 = fma_t(us, y1, su1);
+            su2 = fma_t(us, y2, su2);
+            s00 = fma_t(y0, y0, s00); s01 = fma_t(y0, y1, s01);
+            s02 = fma_t(y0, y2, s02); s11 = fma_t(y1, y1, s11);
+            s12 = fma_t(y1, y2, s12); s22 = fma_t(y2, y2, s22);
         }
     const double inv = 1.0 / (double)n.get();
     const double var0d = suu - su * su * inv;
@@ -317,39 +498,74 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             // frame's parabola stays inside [2, max(TIn) - 2] (the integer
             // reach above; the margin covers the FP32 evaluation), the cast of
             // agree.hpp:164 is the identity and every interpolated sample
-            // moves by at most 0.51 (0.5 plus the FP32 evaluation error):
-            // |e| <= delta = 0.51 sqrt(n). Then with s = sqrt(var1),
-            // covariance and norm move by at most sqrt(var0) delta and delta,
-            // the correlation by at most 2 delta / (s - delta) (+1e-4 for the
-            // FP32 moments and arithmetic of both forms and the exact versus
-            // two-pass left variance), and var1 itself by at most
-            // 2 s delta + delta^2 (+1e-4 var1).
+            // moves by at most 0.52 (0.5 for the rounding plus the FP32
+            // evaluation error of ((a x) x + b x) + c, whose four roundings
+            // are each at most half an ulp of 65536, 0.0039):
+            // |e| <= delta = 0.52 sqrt(n). Then with s = sqrt(var1), the
+            // correlation moves by at most `bound` below, and var1 itself by
+            // at most 2 s delta + delta^2 (+1e-4 var1 + 1 for the reference's
+            // FP32 two-pass variance).
             constexpr float VMAX = (float)(TIn)~(TIn)0;
-            const float delta = 0.51f * __builtin_sqrtf((float)n.get());
+            const float delta = 0.52f * __builtin_sqrtf((float)n.get());
             float best_x = 0.0f, best = -1.0f, eps_best = 2.0f;
             float upper = -1.0f;  // max over x of closed form + bound
+            float lower = -1.0f;  // max over x of closed form - bound
+            // BX: largest upper bounds h1 >= h2 >= h3 (at points i1, i2)
+            float h1 = -2.0f, h2 = -2.0f, h3 = -2.0f;
+            int i1 = 0, i2 = 0, k = 0;
             bool settled = ylo >= 2 && yhi <= (int)VMAX - 2;
             const float rv0 = __builtin_amdgcn_rsqf((float)var0d);
             // The closed form is evaluated with fused multiply-adds (its FP32
             // error is covered by the 1e-4 above); this file is otherwise
             // compiled without contraction for the reference's arithmetic.
             const float ab2 = 2.0f * ab, bbac = bb + 2.0f * ac, bc2 = 2.0f * bc;
-            const float dd2 = 2.0f * delta, dsq1 = __builtin_fmaf(delta, delta, 1.0f);
+            // FP32 evaluation error of the closed forms on |x| <= 1: the
+            // coefficients' FP32 rounding plus the fused Horner steps, each
+            // at most half an ulp of a partial sum bounded by the sum of the
+            // coefficients' magnitudes, so below (5 + 4) 2^-24 = 5.4e-7 of
+            // that sum (1e-6 is used): covariance cerr, var1 verr (absolute).
+            // Where var1(x) is small against its coefficients this term
+            // dominates.
+            const float cerr = 1e-6f * (fabsf(ua) + fabsf(ub) + fabsf(uc));
+            const float verr = 1e-6f * (fabsf(aa) + fabsf(ab2) + fabsf(bbac) + fabsf(bc2) + fabsf(cc));
+            const float cerr_r0 = cerr * rv0;
+            const float dd2 = 2.0f * delta, dsq1 = __builtin_fmaf(delta, delta, 1.0f) + verr;
             const auto cov_at = [&](float x) {
                 return __builtin_fmaf(__builtin_fmaf(ua, x, ub), x, uc);
             };
             const auto v1_at = [&](float x) {
                 return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(__builtin_fmaf(aa, x, ab2), x, bbac), x, bc2), x, cc);
             };
-            const auto bound = [&](float sv) {
-                return __builtin_fmaf(2.000001f * delta, __builtin_amdgcn_rcpf(sv - delta), 1e-4f);
+            // Correlation bound at a point with sqrt(var1) = sv > delta,
+            // r1 = 1 / sv and closed-form correlation nxc. Rounding the
+            // samples adds a centred e, |e| <= delta, to the centred right
+            // vector y (|y| = s); with p, q the unit vectors of the centred
+            // left samples and of y, r = <p, q>:
+            //   r' - r = (<p - r q, e> - r <q, e> (s - s') / (s + s') - r |e|^2 / (s + s')) / s'
+            // and |p - r q| = sqrt(1 - r^2), so
+            //   |r' - r| <= (sqrt(1 - r^2) delta + delta^2 / (s - delta)) / (s - delta).
+            // sqrt(1 - r^2) is taken from nxc with 5e-4 of room for its own
+            // error. On top: the reference's FP32 arithmetic on the rounded
+            // samples (BX, exact moments: the covariance and variance chains
+            // of n fused steps err by at most n 2^-24 relative each, the
+            // correlation by below 1.7e-7 n + 1e-6 in all, 1e-6 + 2.5e-7 n
+            // is used; MOMENTS, FP32 moments: 1e-4) and the closed forms'
+            // error (cerr / sqrt(var0 var1) + verr / var1).
+            const float slack = BX ? 1e-6f + 2.5e-7f * (float)n.get() : 1e-4f;
+            const float dsq = delta * delta;
+            const auto bound = [&](float sv, float r1, float nxc) {
+                const float inv = 1.000001f * __builtin_amdgcn_rcpf(sv - delta);
+                // v_sqrt_f32 (1 ulp): w only scales the bound
+                const float w = 1.000001f * __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-nxc, nxc, 1.0005f), 0.0f));
+                return __builtin_fmaf(__builtin_fmaf(dsq, inv, w * delta), inv,
+                                      __builtin_fmaf(r1, __builtin_fmaf(verr, r1, cerr_r0), slack));
             };
             for (float x = -1.0f; x <= 1.0f; x += step) {
                 const float cov = cov_at(x);
                 const float v1 = v1_at(x);
                 const float r1 = __builtin_amdgcn_rsqf(v1);
-                const float sv = v1 * r1;  // sqrt(var1)
-                const float margin = __builtin_fmaf(dd2, sv, __builtin_fmaf(1e-4f, v1, dsq1));
+                const float sv = v1 > 0.0f ? v1 * r1 : 0.0f;  // sqrt(var1)
+                const float margin = __builtin_fmaf(dd2, sv, __builtin_fmaf(1e-4f, fabsf(v1), dsq1));
                 float nxc = -1.0f, eps = 0.0f;
                 if (mv && v1 < minvar) {
                     // below the variance test: also exactly when below by the margin
@@ -358,26 +574,62 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     if (mv && v1 < minvar + margin) settled = false;
                     if (!(sv > 2.0f * delta)) settled = false;  // no useful bound here
                     nxc = cov * r1 * rv0;
-                    eps = bound(sv);
+                    eps = bound(sv, r1, nxc);
                 }
-                upper = fmaxf(upper, nxc + eps);
-                if (best < nxc) {
-                    best = nxc;
-                    best_x = x;
-                    eps_best = eps;
+                if (!(fabsf(nxc) <= 1.0002f)) settled = false;  // no usable closed form
+                const float hi = nxc + eps;
+                lower = fmaxf(lower, nxc - eps);
+                if constexpr (BX) {
+                    // the three largest upper bounds, with the points of the
+                    // first two (v_med3 keeps the order statistics)
+                    const bool g1 = hi > h1, g2 = hi > h2;
+                    i2 = g1 ? i1 : (g2 ? k : i2);
+                    i1 = g1 ? k : i1;
+                    h3 = fmaxf(fminf(hi, h2), h3);
+                    h2 = fmaxf(fminf(hi, h1), h2);
+                    h1 = fmaxf(h1, hi);
+                    ++k;
+                } else {
+                    upper = fmaxf(upper, hi);
+                    if (best < nxc) {
+                        best = nxc;
+                        best_x = x;
+                        eps_best = eps;
+                    }
                 }
             }
+            if constexpr (BX) {
+                // Candidates are the points whose upper bound reaches `lower`
+                // (the point attaining `lower` is one of them). Only the first
+                // (h2 < lower) or the first two (h3 < lower) upper bounds can
+                // qualify; a wider set is scored in full.
+                const uint64_t all = npts >= 64 ? ~0ull : (1ull << npts) - 1ull;
+                if (!settled) {
+                    cmask = all;
+                    st |= ST_PENDING;
+                } else if (!(fmaxf(h1, -1.0f) < thr)) {  // the reference's best starts at -1
+                    if (h2 < lower) {
+                        // one candidate: its exact value is at least `lower`
+                        // and above every other point's
+                        if (lower >= thr && lower > -1.0f) res = (float)d - gx->x[i1];
+                        else cmask = 1ull << i1;
+                    } else {
+                        cmask = h3 < lower ? (1ull << i1) | (1ull << i2) : all;
+                    }
+                    if (cmask) st |= ST_PENDING;
+                }
+            } else if (settled && best - eps_best >= thr) {
             // Accepted with the closed-form grid point when the bound settles
             // it and no grid point half a pixel or more away could beat the
             // winner; else the exact stage.
-            if (settled && best - eps_best >= thr) {
                 float far = -1.0f;
                 for (float x = -1.0f; x <= 1.0f; x += step) {
                     if (fabsf(x - best_x) < 0.5f) continue;
                     const float v1 = v1_at(x);
                     const float r1 = __builtin_amdgcn_rsqf(v1);
                     if (mv && v1 < minvar) continue;  // settled: exactly -1 there
-                    far = fmaxf(far, __builtin_fmaf(cov_at(x) * r1, rv0, bound(v1 * r1)));
+                    const float nxc = cov_at(x) * r1 * rv0;
+                    far = fmaxf(far, nxc + bound(v1 * r1, r1, nxc));
                 }
                 if (far < best - eps_best)
                     res = (float)d - best_x;
@@ -389,11 +641,22 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
         }
     }
     const uint64_t pend = __ballot(act && (st & ST_PENDING));
-    if (pend)
-        exact_pending_wave<TIn>(pend, row, col, d, f0, f1, n, thr, step, npts, mv, minvar, res, st, xs);
-    if (act) {
-        out[(size_t)row * ostep + col] = res;
-        status[(size_t)row * W + col] = st;
+    if constexpr (BX) {
+        if (pend) {
+            const float mean0 = (float)su / (float)n.get();  // su: exact, below 2^24
+            const bool inrange = ylo >= 2 && yhi <= (int)(TIn)~(TIn)0 - 2;
+            exact_candidates_wave<TIn>(act && (st & ST_PENDING), cmask, row, col, d, mean0, inrange,
+                                       f0, f1, n, thr, *gx, mv, minvar, res, *(CandLds<NMAX>*)lds);
+        }
+        if (act) out[(size_t)row * ostep + col] = res;
+    } else {
+        if (pend)
+            exact_pending_wave<TIn>(pend, row, col, d, f0, f1, n, thr, step, npts, mv, minvar, res, st,
+                                    (float4*)lds);
+        if (act) {
+            out[(size_t)row * ostep + col] = res;
+            status[(size_t)row * W + col] = st;
+        }
     }
 }
 
@@ -409,6 +672,22 @@ __global__ __launch_bounds__(256) void k_agree_subpixel_moments(
     agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(col < W, row, min(col, W - 1), disp, dstep, f0, f1,
                                                   n_rt, W, thr, step, npts, has_minvar, minvar, out,
                                                   ostep, status, xs[threadIdx.x >> 6]);
+}
+
+// Bit-exact subpixel agreement (Precision::SINGLE, no corrmap, npts <= 64):
+// closed-form pruning, the exact stage on the candidate points only.
+template<typename TIn, int NMAX, int N_EXACT>
+__global__ __launch_bounds__(256) void k_agree_subpixel_pruned(
+    const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1, int n_rt,
+    int W, float thr, float step, int npts, int has_minvar, float minvar, float* __restrict__ out,
+    int64_t ostep, const GridX gx) {
+    __shared__ CandLds<NMAX> cl[4];
+    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
+    if ((col & ~63) >= W)  // whole wave past the row
+        return;
+    agree_subpixel_moments_px<TIn, NMAX, N_EXACT, true>(col < W, row, min(col, W - 1), disp, dstep, f0, f1,
+                                                        n_rt, W, thr, step, npts, has_minvar, minvar, out,
+                                                        ostep, nullptr, &cl[threadIdx.x >> 6], &gx);
 }
 
 // over listed pixels only (dense numbering of the lists; each wave walks

@@ -284,16 +284,19 @@ int run_search_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool
     constexpr size_t lds = (size_t)CH * (2 * WORDS + 1) * 16;
     static_assert(lds <= 64 * 1024, "LDS chunk too large");
     const dim3 grid((W + WAVES * NB * 32 - 1) / (WAVES * NB * 32), H), block(WAVES * 64);
-#define SM(ND, M) LAUNCH("search_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep, (const int16_t*)nullptr, (const int*)nullptr, (int16_t*)nullptr)
+    // keys straight from the matrix product where the column fits its field
+#define SM(ND, M, E) LAUNCH("search_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH, false, E>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep, (const int16_t*)nullptr, (const int*)nullptr, (int16_t*)nullptr)
+#define SM2(ND, M) do { if (W <= EMBED_MAX_W) SM(ND, M, true); else SM(ND, M, false); } while (0)
     if (nodupes) {
-        if (mode == SEARCH_DISP) SM(true, SEARCH_DISP);
-        else if (mode == SEARCH_REV) SM(true, SEARCH_REV);
-        else SM(true, SEARCH_LR);
+        if (mode == SEARCH_DISP) SM2(true, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SM2(true, SEARCH_REV);
+        else SM2(true, SEARCH_LR);
     } else {
-        if (mode == SEARCH_DISP) SM(false, SEARCH_DISP);
-        else if (mode == SEARCH_REV) SM(false, SEARCH_REV);
-        else SM(false, SEARCH_LR);
+        if (mode == SEARCH_DISP) SM2(false, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SM2(false, SEARCH_REV);
+        else SM2(false, SEARCH_LR);
     }
+#undef SM2
 #undef SM
     return BICOS_OK;
 }
@@ -402,6 +405,28 @@ int run_subpixel_half_t(const int16_t* disp, int64_t dstep, const Stack& f0,
     });
 }
 
+// Subpixel kernel at Precision::SINGLE: pruned (default) or full scan.
+int g_subpixel_impl = BICOS_SUBPIXEL_PRUNED;
+
+template<typename TIn>
+int run_subpixel_pruned_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                          uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
+                          float* out, int64_t ostep, hipStream_t s) {
+    GridX gx;
+    const int npts = subpixel_points(step);
+    float x = -1.0f;  // the reference's running sum (agree.hpp:160)
+    for (int k = 0; k < 64; ++k) {
+        gx.x[k] = k < npts ? x : 0.0f;
+        x += step;
+    }
+    return with_nmax(n, [&](auto NM, auto NE) -> int {
+        constexpr int NMAX = decltype(NM)::value, NEX = decltype(NE)::value;
+        LAUNCH("agree_subpixel_pruned", (k_agree_subpixel_pruned<TIn, NMAX, NEX>), dim3((W + 255) / 256, H), dim3(256), 0, s,
+               disp, dstep, f0, f1, (int)n, W, thr, step, npts, has_mv, mv, out, ostep, gx);
+        return BICOS_OK;
+    });
+}
+
 template<typename TIn>
 int run_subpixel_moments_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                            uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
@@ -487,6 +512,14 @@ int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stac
     }
     if (precision == BICOS_PRECISION_MOMENTS)  // a corrmap is written: the exact stage throughout
         precision = BICOS_PRECISION_SINGLE;
+    if (precision == BICOS_PRECISION_SINGLE && !corr && g_subpixel_impl == BICOS_SUBPIXEL_PRUNED) {
+        const int npts = subpixel_points(step);
+        if (npts >= 1 && npts <= 64) {
+            return depth == BICOS_8U
+                ? run_subpixel_pruned_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s)
+                : run_subpixel_pruned_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s);
+        }
+    }
     if (precision == BICOS_PRECISION_HALF_MFMA) {
         return depth == BICOS_8U
             ? run_subpixel_half_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, (float*)corr, cstep, s)
@@ -760,6 +793,13 @@ int bicos_hip_set_search_impl(int32_t impl) {
     if (impl != BICOS_SEARCH_MFMA && impl != BICOS_SEARCH_VALU)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown search implementation");
     g_search_impl = impl;
+    return BICOS_OK;
+}
+
+int bicos_hip_set_subpixel_impl(int32_t impl) {
+    if (impl != BICOS_SUBPIXEL_PRUNED && impl != BICOS_SUBPIXEL_SCAN)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown subpixel implementation");
+    g_subpixel_impl = impl;
     return BICOS_OK;
 }
 

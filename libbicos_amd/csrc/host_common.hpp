@@ -100,6 +100,8 @@ int check_out(const bicos_image* img, int rows, int cols, int type, const char* 
 
 // Row-search implementation (BICOS_SEARCH_MFMA or BICOS_SEARCH_VALU).
 extern int g_search_impl;
+// Subpixel kernel at Precision::SINGLE (BICOS_SUBPIXEL_PRUNED or _SCAN).
+extern int g_subpixel_impl;
 
 int transform_any(const Stack& f0, const Stack* f1, int depth, uint32_t n, int mode,
                   int W, int H, int words, uint32_t* out0, uint32_t* out1, hipStream_t s);

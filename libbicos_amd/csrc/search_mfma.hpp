@@ -10,9 +10,10 @@
 //                         = popcount(c ^ q) - popcount(q)
 //
 // popcount(q) is constant per query, so the matrix product ranks candidates
-// exactly like the Hamming cost. Each output then costs three VALU ops:
-// key = (dot << 16) + column (v_lshl_add_u32), second-smallest key
-// (v_med3_u32, no-duplicates only) and smallest key (v_min_u32); keys are
+// exactly like the Hamming cost. For rows up to 4096 columns the product is
+// scaled and offset so that it is the candidate's sort key itself (cost,
+// then column); each output then costs two VALU ops: second-smallest key
+// (v_med3_i32, no-duplicates only) and smallest key (v_min_i32). Keys are
 // unique per column, so the first strict minimum and the duplicate test come
 // out as in the reference scan, independent of evaluation order.
 //
@@ -55,23 +56,65 @@ __device__ __forceinline__ v4i expand_pm1(uint32_t b16) {
     return r;
 }
 
-// "no candidate" key: above every real key (< 2^26) and far enough below 2^32
-// that the state offsets (at most 32 * 1024 tiles + 4) cannot wrap it around.
-constexpr uint32_t NO_KEY = 0x7FFFFFFFu;
+// 16 descriptor bits -> 16 int8 entries of 0 / 64 (embedded-key operands)
+__device__ __forceinline__ uint4 expand0_64(uint32_t b16) {
+    const uint4 v = expand01(b16);
+    return make_uint4(v.x << 6, v.y << 6, v.z << 6, v.w << 6);
+}
 
+// 16 descriptor bits -> 16 int8 entries of +64 (bit 0) / -64 (bit 1)
+__device__ __forceinline__ v4i expand_pm64(uint32_t b16) {
+    v4i r;
+    r[0] = (int)((spread4(b16) << 7) | 0x40404040u);
+    r[1] = (int)((spread4(b16 >> 4) << 7) | 0x40404040u);
+    r[2] = (int)((spread4(b16 >> 8) << 7) | 0x40404040u);
+    r[3] = (int)((spread4(b16 >> 12) << 7) | 0x40404040u);
+    return r;
+}
+
+__device__ __forceinline__ int imed3(int a, int b, int c) {
+    // lowered to v_med3_i32
+    return max(min(a, b), min(max(a, b), c));
+}
+
+// "no candidate" key: above every real key (|key| < 2^21 in both key forms)
+// and far enough below 2^31 that the state offsets (at most 32 * 1024 tiles)
+// cannot wrap it around.
+constexpr int NO_KEY = 0x3FFFFFFF;
+
+// Largest row width of the embedded-key form: the column field is 12 bits.
+constexpr int EMBED_MAX_W = 4096;
+
+// Keys. A key orders candidates by (cost, column) and is unique per column.
+//
+// EMBED = false: key = ((dot + 256) << 16) + column, one v_lshl_add_u32 per
+//   output after the products (query entries +1 / -1, candidate entries 0 / 1).
+// EMBED = true (rows up to 4096 columns): the matrix product itself yields
+//   the key. Query entries are +64 / -64 and candidate entries 0 / 64, so the
+//   product is 4096 * dot; the accumulator of the first K step starts from the
+//   candidate's row within the 32-column tile instead of 0. The sum is
+//   4096 * dot + row, exact in int32, with no VALU work: each output then
+//   costs two VALU ops (v_med3_i32, v_min_i32) instead of three.
+//
+// In both forms the running state is kept relative to the current tile
+// (minus 32 per tile), so the tile base never has to be added per output.
+//
 // LIST = true: the queries of row r are the listed columns
 // qcol[r * W + i], i < row_count[r] (negative entries are skipped); the row
 // is blockIdx.x so that rows, most of which need one block, spread over the
 // XCDs. Results go to out[r * ostep + col] (SEARCH_DISP) or to
 // res[r * W + i] (SEARCH_REV). Waves without listed queries skip the
 // arithmetic but take part in the staging.
-template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH, bool LIST = false>
+template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH, bool LIST = false,
+         bool EMBED = false>
 __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
     const uint32_t* __restrict__ dq, const uint32_t* __restrict__ dc, int W,
     const int16_t* __restrict__ rev, int max_lr_diff, int16_t* __restrict__ out, int64_t ostep,
     const int16_t* __restrict__ qcol, const int* __restrict__ row_count, int16_t* __restrict__ res) {
     constexpr int KS = WORDS;              // K steps of 32 bits
     constexpr int STRIDE = 2 * KS + 1;     // uint4 per staged candidate (16 B pad)
+    constexpr int CSHIFT = EMBED ? 12 : 16;           // column field width of a key
+    constexpr int CMASK = (1 << CSHIFT) - 1;
     extern __shared__ uint4 lds[];
 
     const int row = LIST ? blockIdx.x : blockIdx.y;
@@ -87,7 +130,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
 
     // ---- query operands: bits 32 s + 16 g .. +15 of each query pixel
     v4i qop[NB][KS];
-    uint32_t s1[NB], s2[NB];
+    int s1[NB], s2[NB];
     int qc[NB];
     const int qbase = (group * WAVES + wave) * (NB * 32);
 #pragma unroll
@@ -97,11 +140,18 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
         qc[b] = col;
         const uint32_t* src = dq + ((size_t)row * W + (col >= 0 && col < W ? col : 0)) * WORDS;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) qop[b][s] = expand_pm1(src[s] >> (16 * g));
+        for (int s = 0; s < KS; ++s)
+            qop[b][s] = EMBED ? expand_pm64(src[s] >> (16 * g)) : expand_pm1(src[s] >> (16 * g));
         s1[b] = NO_KEY;
         s2[b] = NO_KEY;
     }
     const bool busy = !LIST || qbase < count;
+
+    // candidate row of accumulator i within a tile (the C operand of the
+    // first K step in the embedded form)
+    v16i crow_init;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) crow_init[i] = EMBED ? 8 * (i / 4) + 4 * g + (i % 4) : 0;
 
     const uint32_t* crow = dc + (size_t)row * W * WORDS;
     const int ntiles = (W + 31) / 32;
@@ -113,13 +163,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
             const int m = it / (2 * KS), c = it - m * (2 * KS);
             uint32_t bits = 0;
             if (m < cn) bits = (crow[(size_t)(c0 + m) * WORDS + (c >> 1)] >> (16 * (c & 1))) & 0xFFFFu;
-            lds[m * STRIDE + c] = expand01(bits);
+            lds[m * STRIDE + c] = EMBED ? expand0_64(bits) : expand01(bits);
         }
         __syncthreads();
 
-        // full tiles: three VALU ops per output; the NB product chains are
-        // independent, so matrix-core work of one overlaps the VALU work of
-        // the previous
+        // full tiles; the NB product chains are independent, so matrix-core
+        // work of one overlaps the VALU work of the previous
         const int nfull = busy ? cn / 32 : 0;
         for (int mt = 0; mt < nfull; ++mt) {
             v4i aop[KS];
@@ -131,7 +180,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
             v16i acc[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], v16i{0}, 0, 0, 0);
+                acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], crow_init, 0, 0, 0);
 #pragma unroll
                 for (int s = 1; s < KS; ++s)
                     acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[s], qop[b][s], acc[b], 0, 0, 0);
@@ -141,13 +190,16 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     // candidate row within the tile: 8 (i / 4) + 4 g + i % 4; the
-                    // 4 g part and the tile base live in the running state offset
-                    const uint32_t key = ((uint32_t)acc[b][i] << 16) + ((256u << 16) + 8u * (i / 4) + (i % 4));
-                    if constexpr (NODUPES) s2[b] = umed3(s1[b], s2[b], key);
+                    // tile base (and 4 g in the shifted form) live in the running
+                    // state offset
+                    const int key = EMBED ? acc[b][i]
+                                          : (int)(((uint32_t)acc[b][i] << 16) +
+                                                  ((256u << 16) + 8u * (i / 4) + (i % 4)));
+                    if constexpr (NODUPES) s2[b] = imed3(s1[b], s2[b], key);
                     s1[b] = min(s1[b], key);
                 }
-                s1[b] -= 32u;
-                if constexpr (NODUPES) s2[b] -= 32u;
+                s1[b] -= 32;
+                if constexpr (NODUPES) s2[b] -= 32;
             }
         }
         // ragged last tile of the row: candidates at or beyond W are masked
@@ -162,39 +214,42 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
             const int tbase = c0 + mt * 32;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                v16i acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], v16i{0}, 0, 0, 0);
+                v16i acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[0], qop[b][0], crow_init, 0, 0, 0);
 #pragma unroll
                 for (int s = 1; s < KS; ++s)
                     acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[s], qop[b][s], acc, 0, 0, 0);
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    uint32_t key = ((uint32_t)acc[i] << 16) + ((256u << 16) + 8u * (i / 4) + (i % 4));
+                    int key = EMBED ? acc[i]
+                                    : (int)(((uint32_t)acc[i] << 16) +
+                                            ((256u << 16) + 8u * (i / 4) + (i % 4)));
                     if (tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = NO_KEY;
-                    if constexpr (NODUPES) s2[b] = umed3(s1[b], s2[b], key);
+                    if constexpr (NODUPES) s2[b] = imed3(s1[b], s2[b], key);
                     s1[b] = min(s1[b], key);
                 }
-                s1[b] -= 32u;
-                if constexpr (NODUPES) s2[b] -= 32u;
+                s1[b] -= 32;
+                if constexpr (NODUPES) s2[b] -= 32;
             }
         }
     }
 
-    // restore absolute keys: + 32 * (tiles scanned) + 4 g, then merge lanes l, l ^ 32
-    const uint32_t off = 32u * (uint32_t)ntiles + 4u * (uint32_t)g;
+    // restore absolute keys: + 32 * (tiles scanned) (+ 4 g in the shifted
+    // form), then merge lanes l, l ^ 32
+    const int off = 32 * ntiles + (EMBED ? 0 : 4 * g);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        uint32_t b1 = s1[b] + off, b2 = s2[b] + off;
-        const uint32_t o1 = (uint32_t)__shfl_xor((int)b1, 32);
-        const uint32_t o2 = (uint32_t)__shfl_xor((int)b2, 32);
-        const uint32_t m1 = min(b1, o1);
-        const uint32_t m2 = min(max(b1, o1), min(b2, o2));
+        const int b1 = s1[b] + off, b2 = s2[b] + off;
+        const int o1 = __shfl_xor(b1, 32);
+        const int o2 = __shfl_xor(b2, 32);
+        const int m1 = min(b1, o1);
+        const int m2 = min(max(b1, o1), min(b2, o2));
         const int col = qc[b];
         if constexpr (LIST) {
             const int i = qbase + b * 32 + nl;
             if (g != 0 || i >= count)
                 continue;
-            const bool valid = col >= 0 && !(NODUPES && (m1 >> 16) == (m2 >> 16));
-            const int16_t best = valid ? (int16_t)(m1 & 0xFFFFu) : INVALID16;
+            const bool valid = col >= 0 && !(NODUPES && (m1 >> CSHIFT) == (m2 >> CSHIFT));
+            const int16_t best = valid ? (int16_t)(m1 & CMASK) : INVALID16;
             if constexpr (MODE == SEARCH_REV)
                 res[(size_t)row * W + i] = best;
             else if (col >= 0)
@@ -203,8 +258,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
         }
         if (g != 0 || col >= W)
             continue;
-        const int best = (int)(m1 & 0xFFFFu);
-        const bool valid = !(NODUPES && (m1 >> 16) == (m2 >> 16));
+        const int best = m1 & CMASK;
+        const bool valid = !(NODUPES && (m1 >> CSHIFT) == (m2 >> CSHIFT));
         int16_t v = INVALID16;
         if (valid) {
             if constexpr (MODE == SEARCH_DISP) {

@@ -307,12 +307,14 @@ int run_search_list_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W,
                            int16_t* disp, int64_t dstep, hipStream_t s) {
     constexpr size_t lds = (size_t)CH * (2 * WORDS + 1) * 16;
     const dim3 grid(H, (W + WAVES * NB * 32 - 1) / (WAVES * NB * 32)), block(WAVES * 64);
-#define SM(ND, M) LAUNCH("search_list_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH, true>), grid, block, lds, s, dq, dc, W, (const int16_t*)nullptr, 0, disp, dstep, qcol, row_count, res)
+#define SM(ND, M, E) LAUNCH("search_list_mfma", (k_search_mfma<WORDS, NB, ND, M, WAVES, CH, true, E>), grid, block, lds, s, dq, dc, W, (const int16_t*)nullptr, 0, disp, dstep, qcol, row_count, res)
+#define SM2(ND, M) do { if (W <= EMBED_MAX_W) SM(ND, M, true); else SM(ND, M, false); } while (0)
     if (nodupes) {
-        if (mode == SEARCH_REV) SM(true, SEARCH_REV); else SM(true, SEARCH_DISP);
+        if (mode == SEARCH_REV) SM2(true, SEARCH_REV); else SM2(true, SEARCH_DISP);
     } else {
-        if (mode == SEARCH_REV) SM(false, SEARCH_REV); else SM(false, SEARCH_DISP);
+        if (mode == SEARCH_REV) SM2(false, SEARCH_REV); else SM2(false, SEARCH_DISP);
     }
+#undef SM2
 #undef SM
     return BICOS_OK;
 }

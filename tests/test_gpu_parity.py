@@ -221,6 +221,75 @@ def test_agree_subpixel_exact(gpu, oracle, dtype, double):
             assert same(got_cm, exp_cm), (n, step, minvar)
 
 
+@pytest.fixture
+def scan_subpixel(gpu):
+    """the full-scan subpixel kernel for one test, then back to the default"""
+    gpu.set_subpixel_impl(gpu.SUBPIXEL_SCAN)
+    yield
+    gpu.set_subpixel_impl(gpu.SUBPIXEL_PRUNED)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_agree_subpixel_pruned_exact(gpu, oracle, dtype):
+    """Precision::SINGLE without a corrmap runs the pruned kernel: bit-exact
+    against the oracle on smooth rows (multi-candidate peaks), saturated rows
+    (parabolas leaving the sample range: every grid point scored), constant
+    and low-variance pixels, thresholds down to -1, and grid sizes 3..64
+    (101 points fall back to the full scan)."""
+    rng = np.random.default_rng(3100 + (dtype == np.uint16))
+    hi = np.iinfo(dtype).max
+    cases = ((10, 0.25, "smooth"), (33, 0.1, "smooth"), (12, 0.05, "smooth"), (64, 0.05, "smooth"),
+             (5, 0.3, "smooth"), (33, 0.1, "saturated"), (20, 0.1, "flat"), (33, 0.02, "smooth"),
+             (2, 0.5, "smooth"), (40, 0.0317, "smooth"))
+    for n, step, kind in cases:
+        rows, cols = 12, int(rng.integers(700, 1300))
+        x = np.arange(cols + 8, dtype=np.float64)
+        base = np.empty((n, rows, cols + 8))
+        for t in range(n):
+            for r in range(rows):
+                ph, fr = rng.uniform(0, 6.3, 3), rng.uniform(0.05, 0.6, 3)
+                base[t, r] = sum(np.sin(fr[i] * x + ph[i]) for i in range(3))
+        base = (base - base.min()) / (base.max() - base.min())
+        if kind == "saturated":
+            base = np.clip(base * 1.6 - 0.3, 0, 1)
+        base = np.rint(base * hi).astype(dtype)
+        if kind == "flat":   # constant and nearly constant pixels
+            base[:, :, ::7] = base[0:1, :, ::7]
+            base[:, :, 3::7] = (base[0:1, :, 3::7].astype(np.int64)
+                                + rng.integers(0, 2, (n, rows, 1))).clip(0, hi)
+        shift = 2
+        s0 = base[:, :, 4:cols + 4].copy()
+        s1 = base[:, :, 4 - shift:cols + 4 - shift].copy()
+        # noise so that correlations are not all 1
+        noise = rng.normal(0, hi * 0.004, s1.shape)
+        s1 = np.clip(np.rint(s1 + noise), 0, hi).astype(dtype)
+        disp = np.full((rows, cols), shift, np.int16)
+        mix = rng.random((rows, cols))
+        disp[mix < 0.2] = rng.integers(-1, cols, size=int((mix < 0.2).sum()))
+        disp[mix > 0.95] = -32768
+        disp[:, 0] = 0
+        disp[:, -1] = 0
+        for thr in (float(rng.uniform(0.3, 0.95)), 0.999, -1.0):
+            for minvar in (float(rng.uniform(0, 5)), None):
+                mv_n = None if minvar is None else np.float32(minvar) * np.float32(n)
+                exp, _ = oracle.agree_subpixel(disp, s0, s1, thr, step, mv_n)
+                got = host(gpu.agree_subpixel(dev(disp), dev(s0), dev(s1), thr, step, minvar, 0))
+                bad = np.argwhere(~((got == exp) | (np.isnan(got) & np.isnan(exp))))
+                assert same(got, exp), (n, step, kind, thr, minvar, bad[:5])
+
+
+def test_agree_subpixel_scan_knob(gpu, oracle, scan_subpixel):
+    """the full-scan kernel stays selectable and agrees with the oracle"""
+    rng = np.random.default_rng(3200)
+    n, rows, cols = 33, 8, 900
+    s0 = rand_stack(rng, n, rows, cols, np.uint16)
+    s1 = np.roll(s0, 3, axis=2)
+    disp = np.full((rows, cols), 3, np.int16)
+    exp, _ = oracle.agree_subpixel(disp, s0, s1, 0.5, 0.1, None)
+    got = host(gpu.agree_subpixel(dev(disp), dev(s0), dev(s1), 0.5, 0.1, None, 0))
+    assert same(got, exp)
+
+
 # ----------------------------------------------------------- whole path
 
 def run_match(gpu, L, R, cfg):
@@ -259,6 +328,8 @@ def test_match_c1_band_u16_limited_subpixel(gpu, oracle):
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
     assert got.dtype == np.float32
     assert same(got, exp) and same(got_cm, exp_cm)
+    d, _ = gpu.match_device(dev(L), dev(R), cfg, want_corrmap=False)  # pruned subpixel
+    assert same(host(d), exp)
     v = ~np.isnan(got)
     assert v.mean() > 0.5 and (np.abs(got[v] - gt[v]) <= 1).mean() > 0.95
 
@@ -291,6 +362,8 @@ def test_match_c4_band_u16_64frames_consistency(gpu, oracle, search_impl):
     got, got_cm = run_match(gpu, L, R, cfg)
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
     assert same(got, exp) and same(got_cm, exp_cm)
+    d, _ = gpu.match_device(dev(L), dev(R), cfg, want_corrmap=False)  # pruned subpixel
+    assert same(host(d), exp)
     v = ~np.isnan(got)
     assert v.mean() > 0.5 and (np.abs(got[v] - gt[v]) <= 1).mean() > 0.95
 
@@ -318,6 +391,9 @@ def test_match_full_size_c1(gpu, oracle):
     got, got_cm = run_match(gpu, L, R, cfg)
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
     assert same(got, exp) and same(got_cm, exp_cm)
+    # without a corrmap the subpixel stage is the pruned kernel
+    d, cm = gpu.match_device(dev(L), dev(R), cfg, want_corrmap=False)
+    assert cm is None and same(host(d), exp)
     v = ~np.isnan(got)
     print(f"valid {v.mean():.4f}, within 1px of ground truth {(np.abs(got[v] - gt[v]) <= 1).mean():.4f}")
 
