@@ -29,6 +29,8 @@ METRIC = "matched Mpixel/s @1920x1200x33 u16; learned vs plain at >=99.5% px agr
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
 INT8_DENSE_TOPS = 5000.0
+VALU_SIMDS = 1024        # 256 CUs x 4 SIMDs; a wave64 VALU instruction takes 4 cycles
+CLOCK_HZ = 2.4e9
 
 
 def parse_args():
@@ -61,6 +63,17 @@ def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int
         return {"bound": "mfma", "achieved": ops / s / 1e12, "peak": INT8_DENSE_TOPS,
                 "unit": "TFLOP/s", "frac": ops / s / 1e12 / INT8_DENSE_TOPS,
                 "work_per_launch": ops, "work_unit": "int8-equivalent ops"}
+    if name.startswith("agree_subpixel"):
+        # bound by vector-ALU issue: instructions per wave (rocprofv3
+        # SQ_INSTS_VALU / SQ_WAVES, committed) x waves x 4 cycles against
+        # 1024 SIMDs at 2.4 GHz
+        valu = committed_valu(name)
+        if valu is not None:
+            per_wave, waves = valu
+            issue_s = per_wave * waves * 4 / (VALU_SIMDS * CLOCK_HZ)
+            return {"bound": "valu", "achieved": issue_s / s, "peak": 1.0, "unit": "issue fraction",
+                    "frac": issue_s / s, "work_per_launch": per_wave * waves,
+                    "work_unit": "wave64 VALU instructions"}
     if name.startswith("transform"):
         b = 2 * W * H * (n * 2 + 4 * words)  # one launch transforms both views
     elif name.startswith("agree_subpixel"):
@@ -81,6 +94,19 @@ def committed_traffic(kernel: str):
         m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\d+(?:\.\d+)?)", line)
         if m and kernel.startswith(m.group(1)):
             return float(m.group(2))
+    return None
+
+
+def committed_valu(kernel: str):
+    """(VALU instructions per wave, waves per launch) from rocprofv3 SQ
+    counters, recorded in profiles/pmc_traffic.txt."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"#\s*(\S+)\s+fetch.*_VALU=(\d+).*waves\s+(\d+)", line)
+        if m and kernel == m.group(1):
+            return float(m.group(2)), float(m.group(3))
     return None
 
 
@@ -298,7 +324,9 @@ def main():
     roof["kernel"] = dom
     roof["traffic"] = committed_traffic(dom)
     for k, v in kernels.items():
-        v["roofline_frac"] = round(kernel_roofline(k, v["avg_ms"], W, H, n, words)["frac"], 4)
+        r = kernel_roofline(k, v["avg_ms"], W, H, n, words)
+        v["bound"] = r["bound"]
+        v["roofline_frac"] = round(r["frac"], 4)
 
     d = disp.cpu().numpy()
     valid = float(np.isfinite(d).mean())

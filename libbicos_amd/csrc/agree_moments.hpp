@@ -1,5 +1,7 @@
-// Subpixel agreement in the MOMENTS precision mode (no reference
-// counterpart; the plain mode is k_agree_subpixel, agree.hpp).
+// Subpixel agreement from closed-form moments: the MOMENTS precision mode
+// (no reference counterpart) and the bit-exact pruned form of the plain
+// SINGLE stage (k_agree_subpixel_pruned; the full scan is k_agree_subpixel,
+// agree.hpp).
 //
 // Without the rounding of the interpolated samples (agree.hpp:164) the
 // subpixel correlation is a closed-form function of the moments of the left
@@ -7,20 +9,25 @@
 // with centred cross moments uY_j and Gram matrix G_jk,
 //   covar(x) = x^2 uA + x uB + uC,   uA = (uY0 - 2 uY1 + uY2) / 2, uB = (uY2 - uY0) / 2, uC = uY1
 //   var1(x)  = x^4 AA + 2 x^3 AB + x^2 (BB + 2 AC) + 2 x BC + CC   (from G)
-// One lane per pixel accumulates the integer-valued moments in FP64 (exact:
-// sums of products of 16-bit samples stay below 2^53), evaluates the closed
-// form in FP32 at every grid point and bounds what the left-out rounding can
-// change there. Where the bound settles accept/reject, the pixel is done:
-// the reference's decision, with the closed-form winner as the grid point
-// (which can differ from the exact scan's where two grid points within half
-// a pixel of each other lie within the bound; a farther rival sends the
-// pixel to the exact stage).
-// Every other pixel is pending: before the kernel writes its results, the
-// wave runs the exact FP32 subpixel stage on its pending pixels
-// (exact_pending_wave), one at a time with all 64 lanes, while their samples
-// are still in the L2 cache; so accept/reject is the reference's for every
-// pixel. Samples are streamed from memory (twice where the reference's
-// two-pass sums need them), not held in registers.
+// One lane per pixel accumulates the moments (left moments in FP64; cross
+// moments as FP32 sums of samples shifted by the first frame's in MOMENTS,
+// exact FP64 sums in the pruned form), evaluates the closed form in FP32 at
+// every grid point and bounds what the left-out rounding and the FP32
+// arithmetic can change there.
+//
+// MOMENTS: where the bound settles accept/reject, the pixel is done: the
+// reference's decision, with the closed-form winner as the grid point (which
+// can differ from the exact scan's where two grid points within half a pixel
+// of each other lie within the bound; a farther rival sends the pixel to the
+// exact stage). Every other pixel is pending: before the kernel writes its
+// results, the wave runs the exact FP32 subpixel stage on its pending pixels
+// (exact_pending_wave) while their samples are still in the L2 cache; so
+// accept/reject is the reference's for every pixel.
+//
+// Pruned (bit-exact): the bound picks the grid points that can hold the
+// scan's maximum; a single one that clears the threshold is the result, else
+// exact_candidates_wave scores the candidates with the reference's FP32
+// arithmetic.
 #pragma once
 
 #include <type_traits>
