@@ -7,6 +7,7 @@ Produces, in-tree next to this file:
   libbicos_hip.so    gfx950 kernels + C ABI (include/bicos_hip.h) + C++ API
                      (include/bicos/match.hpp)
   libbicos_scene.so  synthetic scene generator (host C++)
+  libbicos_io.so     PNG unfiltering for the command-line front-end (host C++)
   bicos_cli          small C++ program exercising the C++ API (tests)
 """
 from __future__ import annotations
@@ -81,6 +82,14 @@ def build_scene() -> str:
     return out
 
 
+def build_io() -> str:
+    out = os.path.join(HERE, "libbicos_io.so")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", out + ".tmp",
+          os.path.join(CSRC, "imageio.cpp")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_cli() -> str:
     out = os.path.join(HERE, "bicos_cli")
     _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", "-o", out + ".tmp",
@@ -92,6 +101,7 @@ def build_cli() -> str:
 
 def build() -> None:
     build_scene()
+    build_io()
     build_hip()
     build_cli()
 

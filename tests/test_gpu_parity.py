@@ -442,6 +442,40 @@ def test_cpp_api_cli(gpu, oracle, tmp_path):
     assert same(got, exp) and same(got_cm, exp_cm)
 
 
+def test_python_cli_png_stacks(gpu, oracle, tmp_path):
+    """python -m libbicos_amd.cli on numbered PNG stacks (two-folder form,
+    16-bit): the raw TIFF disparity and correlation map equal the oracle's,
+    and the .xyz point cloud holds the oracle's selection of the reprojected
+    points (src/cli.cpp:190-250)."""
+    import pointcloud as PC
+    from libbicos_amd import imageio, scene
+    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=111, height=24, width=500, frames=20)
+    for side, st in (("l", L), ("r", R)):
+        (tmp_path / side).mkdir()
+        for t in range(st.shape[0]):
+            imageio.write_png(str(tmp_path / side / f"{t}.png"), st[t])
+    Q = np.array([[1, 0, 0, -250.0], [0, 1, 0, -12.0], [0, 0, 0, 800.0], [0, 0, 0.05, 0]])
+    (tmp_path / "q.yml").write_text("%YAML:1.0\n---\nQ: !!opencv-matrix\n   rows: 4\n   cols: 4\n"
+                                    "   dt: d\n   data: [ " + ", ".join(repr(float(v)) for v in Q.ravel())
+                                    + " ]\n")
+    out = tmp_path / "o" / "disp.png"
+    out.parent.mkdir()
+    cmd = [sys.executable, "-m", "libbicos_amd.cli", str(tmp_path / "l"), str(tmp_path / "r"),
+           "--limited", "-t", "0.9", "-s", "0.1", "--corrmap", "-q", str(tmp_path / "q.yml"),
+           "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.1, min_variance=1.0, full=False, corrmap=True)
+    got = imageio.read_tiff(str(tmp_path / "o" / "disp.tiff"))
+    got_cm = imageio.read_tiff(str(tmp_path / "o" / "disp-corrmap.tiff"))
+    assert same(got, exp) and same(got_cm, exp_cm)
+    assert imageio.read_png(str(tmp_path / "o" / "disp.png")).shape == (24, 500, 3)
+    pts_exp = PC.select(exp, PC.reproject(exp, Q))[0]
+    lines = (tmp_path / "o" / "disp.xyz").read_text().splitlines()
+    assert len(lines) == len(pts_exp)
+    assert lines[:50] == [f"{x:g} {y:g} {z:g}" for x, y, z in pts_exp[:50].tolist()]
+
+
 def test_errors_on_device(gpu):
     import torch
     s = torch.zeros((17, 4, 8), dtype=torch.uint8, device="cuda")
