@@ -43,10 +43,11 @@ def parse_args():
     p.add_argument("--seed", type=int, default=0x600DF00D)
     p.add_argument("--search", choices=("mfma", "valu"), default="mfma",
                    help="row-search kernel: int8 matrix cores (default) or vector ALU")
-    p.add_argument("--workload", choices=("c1", "stress"), default="c1",
+    p.add_argument("--workload", choices=("c1", "stress", "kernels"), default="c1",
                    help="c1: the 1920x1200x33 benchmark line (default); stress: the "
                         "4096x3000x64 Consistency workload, plain and learned, with peak "
-                        "device memory")
+                        "device memory; kernels: the stages at the reference's kernel "
+                        "benchmark shapes (3300x2200 random inputs) beside its RTX 4090 times")
     return p.parse_args()
 
 
@@ -201,6 +202,93 @@ def learned_path(B, dL, dR, plain_disp, args, W, H, plain_ms):
     return res
 
 
+# The reference's published kernel times on an RTX 4090 at 3300x2200 random
+# inputs (bench/baselines/cuda-rtx4090.txt; bench/cuda.cu: one view per
+# transform launch at n = the width's largest stack, max_lr_diff 3, agree at
+# n = 10 in FP64 with threshold 0.9 and min_variance*n = 10), in ms; the
+# search rows are the faster of its two kernels.
+RTX4090_MS = {
+    "transform u16 limited u32 (n 9)": 0.2016, "transform u16 limited u64 (n 17)": 0.3786,
+    "transform u16 limited u128 (n 33)": 1.0756, "transform u8 full u32 (n 6)": 0.2028,
+    "transform u8 full u64 (n 8)": 0.3695, "transform u8 full u128 (n 12)": 0.9292,
+    "search u32 nodup": 9.4276, "search u64 nodup": 11.3053, "search u128 nodup": 18.8214,
+    "search u32 consistency": 11.1487, "search u64 consistency": 18.3765,
+    "search u128 consistency": 36.7618, "search u128 nodup+consistency": 18.9464,
+    "agree u16 n10 fp64": 1.9494, "agree u8 n10 fp64": 1.9478,
+}
+
+
+def kernel_benchmarks(args):
+    """Each stage through its C-ABI entry point at the reference's kernel
+    benchmark shapes: kernel time per call over 20 calls (the library's HIP
+    events around its launches), beside the reference's RTX 4090 time. The subpixel stage
+    has no comparable reference number (its bench passes min_variance and
+    step swapped, bench/cuda.cu:175); it is timed at n = 10, step 0.25."""
+    import torch
+    sys.path.insert(0, ROOT)
+    import libbicos_amd as B
+
+    H, W = 2200, 3300
+    g = torch.Generator(device="cuda").manual_seed(args.seed)
+
+    def rand_stack(n, dtype):
+        hi = 256 if dtype == "u8" else 65536
+        t = torch.randint(0, hi, (n, H, W), dtype=torch.int32, device="cuda", generator=g)
+        return t.to(torch.uint8) if dtype == "u8" else t.to(torch.int16).view(torch.uint16)
+
+    def timed(fn, calls=20):
+        # kernel time per call: the library's HIP events around each launch
+        # (all kernels a call launches), after 5 untimed calls
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        B.timing_enable(True)
+        for _ in range(calls):
+            fn()
+        torch.cuda.synchronize()
+        st = B.timing_read()
+        B.timing_enable(False)
+        return sum(total for _, total in st.values()) / calls
+
+    rows = {}
+    for dtype, full, n, words in (("u16", False, 9, 1), ("u16", False, 17, 2), ("u16", False, 33, 4),
+                                  ("u8", True, 6, 1), ("u8", True, 8, 2), ("u8", True, 12, 4)):
+        st = rand_stack(n, dtype)
+        out = torch.empty((H, W, words), dtype=torch.int32, device="cuda")
+        name = f"transform {dtype} {'full' if full else 'limited'} u{32 * words} (n {n})"
+        rows[name] = timed(lambda: B.transform(st, int(full), out=out))
+        del st
+    for words in (1, 2, 4):
+        d0 = torch.randint(-2**31, 2**31 - 1, (H, W, words), dtype=torch.int32, device="cuda", generator=g)
+        d1 = torch.randint(-2**31, 2**31 - 1, (H, W, words), dtype=torch.int32, device="cuda", generator=g)
+        out = torch.empty((H, W), dtype=torch.int16, device="cuda")
+        for label, flags in (("nodup", 1), ("consistency", 2), ("nodup+consistency", 3)):
+            if label == "nodup+consistency" and words != 4:
+                continue
+            rows[f"search u{32 * words} {label}"] = timed(lambda: B.search(d0, d1, flags, 3, out=out))
+        del d0, d1
+    for dtype in ("u16", "u8"):
+        s0, s1 = rand_stack(10, dtype), rand_stack(10, dtype)
+        disp = torch.randint(-1, W, (H, W), dtype=torch.int32, device="cuda", generator=g).to(torch.int16)
+        work = torch.empty_like(disp)
+
+        def agree():
+            work.copy_(disp)  # agree is in place on the disparity
+            B.agree(work, s0, s1, 0.9, 1.0, B.Precision.Double)
+        rows[f"agree {dtype} n10 fp64"] = timed(agree)
+        sub = torch.empty((H, W), dtype=torch.float32, device="cuda")
+        rows[f"agree_subpixel {dtype} n10 step0.25 fp32"] = timed(
+            lambda: B.agree_subpixel(disp, s0, s1, 0.9, 0.25, 1.0, B.Precision.Single, out=sub))
+        rows[f"agree_subpixel {dtype} n10 step0.25 fp64"] = timed(
+            lambda: B.agree_subpixel(disp, s0, s1, 0.9, 0.25, 1.0, B.Precision.Double, out=sub))
+        del s0, s1
+    table = {k: {"ms": round(v, 4), "rtx4090_ms": RTX4090_MS.get(k),
+                 "speedup": round(RTX4090_MS[k] / v, 2) if k in RTX4090_MS else None}
+             for k, v in rows.items()}
+    print(json.dumps({"workload": "stages at 3300x2200 random inputs (reference kernel benchmarks)",
+                      "kernels": table}))
+
+
 def stress(args):
     """4096x3000, 64 frames/side u16 (256-bit descriptors), Consistency
     (max_lr_diff 1), subpixel step 0.05, threshold 0.96, min_variance 2.0:
@@ -273,6 +361,9 @@ def main():
         sys.exit(2)
     if args.workload == "stress":
         stress(args)
+        return
+    if args.workload == "kernels":
+        kernel_benchmarks(args)
         return
 
     import numpy as np
