@@ -285,8 +285,32 @@ def kernel_benchmarks(args):
     table = {k: {"ms": round(v, 4), "rtx4090_ms": RTX4090_MS.get(k),
                  "speedup": round(RTX4090_MS[k] / v, 2) if k in RTX4090_MS else None}
              for k, v in rows.items()}
+
+    # Whole match at the size of the reference README's dataset run (33+33
+    # frames, LIMITED, threshold 0.96, min_variance 2.0, step 0.1, NoDuplicates:
+    # about 44 ms on an RTX 4090, README.md:71,81; 3208x2200 is the size of its
+    # example disparity), on a synthetic scene of the 1920x1200x33 generator
+    # scaled to that size; inputs resident on the device.
+    from libbicos_amd import scene, window as Wn
+    spec = scene.C1_1920x1200_U16
+    Lh, Rh, _ = Wn.scene_device(spec, args.seed, height=2200, width=3208)
+    cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    d, _ = B.match_device(Lh, Rh, cfg)
+    for _ in range(10):
+        d, _ = B.match_device(Lh, Rh, cfg, disparity=d)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        d, _ = B.match_device(Lh, Rh, cfg, disparity=d)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    whole = {"shape": "3208x2200x33 u16 (synthetic)", "ms": round(ms, 3),
+             "Mpixel/s": round(3208 * 2200 / ms / 1e3, 1), "rtx4090_ms_readme": 44.0,
+             "speedup": round(44.0 / ms, 2)}
     print(json.dumps({"workload": "stages at 3300x2200 random inputs (reference kernel benchmarks)",
-                      "kernels": table}))
+                      "kernels": table, "match_readme_shape": whole}))
 
 
 def stress(args):
