@@ -112,6 +112,7 @@ def committed_valu(kernel: str):
 
 
 def cpu_baseline(L, R, cfg_kw):
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the CPU oracle, timed as the baseline
     threads = len(os.sched_getaffinity(0))
@@ -120,9 +121,27 @@ def cpu_baseline(L, R, cfg_kw):
     O.match(L, R, **cfg_kw)
     dt = time.perf_counter() - t0
     n, H, W = L.shape
+    # one thread on a band of 100 rows (every stage is row-local)
+    O.set_threads(1)
+    band = 100
+    t0 = time.perf_counter()
+    O.match(np.ascontiguousarray(L[:, :band]), np.ascontiguousarray(R[:, :band]), **cfg_kw)
+    dt1 = time.perf_counter() - t0
+    O.set_threads(threads)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": W * H / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
             "sample": f"the full {W}x{H}x{n} workload once ({dt:.2f} s wall, "
-                      f"{threads} threads, oracle/bicos_oracle.cpp)"}
+                      f"{threads} threads, oracle/bicos_oracle.cpp)",
+            "single_thread": {"value": W * band / dt1 / 1e6, "unit": "Mpixel/s",
+                              "sample": f"rows 0..{band - 1} of the same pair ({dt1:.2f} s wall)"},
+            "cpu_model": model}
 
 
 def pointcloud_timing(B, disp, args):
