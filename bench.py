@@ -43,11 +43,12 @@ def parse_args():
     p.add_argument("--seed", type=int, default=0x600DF00D)
     p.add_argument("--search", choices=("mfma", "valu"), default="mfma",
                    help="row-search kernel: int8 matrix cores (default) or vector ALU")
-    p.add_argument("--workload", choices=("c1", "stress", "kernels"), default="c1",
-                   help="c1: the 1920x1200x33 benchmark line (default); stress: the "
-                        "4096x3000x64 Consistency workload, plain and learned, with peak "
-                        "device memory; kernels: the stages at the reference's kernel "
-                        "benchmark shapes (3300x2200 random inputs) beside its RTX 4090 times")
+    p.add_argument("--workload", choices=("1920x1200x33", "4096x3000x64", "kernels"),
+                   default="1920x1200x33",
+                   help="1920x1200x33: the benchmark line (default); 4096x3000x64: the "
+                        "Consistency stress workload, plain and learned, with peak device "
+                        "memory; kernels: the stages at the reference's kernel benchmark "
+                        "shapes (3300x2200 random inputs) beside its RTX 4090 times")
     return p.parse_args()
 
 
@@ -311,7 +312,7 @@ def kernel_benchmarks(args):
     # example disparity), on a synthetic scene of the 1920x1200x33 generator
     # scaled to that size; inputs resident on the device.
     from libbicos_amd import scene, window as Wn
-    spec = scene.C1_1920x1200_U16
+    spec = scene.SCENE_1920x1200_U16
     Lh, Rh, _ = Wn.scene_device(spec, args.seed, height=2200, width=3208)
     cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
     d, _ = B.match_device(Lh, Rh, cfg)
@@ -341,7 +342,7 @@ def stress(args):
     sys.path.insert(0, ROOT)
     import libbicos_amd as B
     from libbicos_amd import scene, window as Wn
-    spec = scene.C4_4096x3000_U16
+    spec = scene.SCENE_4096x3000_U16
     W, H, n = spec.width, spec.height, spec.frames
     free0, total = torch.cuda.mem_get_info()
     L, R, _ = Wn.scene_device(spec, args.seed)
@@ -399,10 +400,10 @@ def stress(args):
 def main():
     args = parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md on row-band sharding)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the stages are "
+              "row-local, so N GPUs would each take a band of rows)", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "stress":
+    if args.workload == "4096x3000x64":
         stress(args)
         return
     if args.workload == "kernels":
@@ -416,7 +417,7 @@ def main():
     import libbicos_amd as B
     from libbicos_amd import scene
 
-    spec = scene.C1_1920x1200_U16
+    spec = scene.SCENE_1920x1200_U16
     W, H, n = spec.width, spec.height, spec.frames
     L, R, _ = scene.generate(spec, seed=args.seed)
     cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,

@@ -46,9 +46,9 @@ class SceneSpec:
 
 
 # The workloads named by the benchmark configurations.
-C0_640x480_U8 = SceneSpec(640, 480, 12, 8, 8.0, 72.0, 20.0, 235.0, 2.0, 0.05, 10.0)
-C1_1920x1200_U16 = SceneSpec(1920, 1200, 33, 16, 0.0, 256.0, 1000.0, 60000.0, 150.0, 0.05, 500.0)
-C4_4096x3000_U16 = SceneSpec(4096, 3000, 64, 16, 0.0, 512.0, 1000.0, 60000.0, 150.0, 0.05, 500.0)
+SCENE_640x480_U8 = SceneSpec(640, 480, 12, 8, 8.0, 72.0, 20.0, 235.0, 2.0, 0.05, 10.0)
+SCENE_1920x1200_U16 = SceneSpec(1920, 1200, 33, 16, 0.0, 256.0, 1000.0, 60000.0, 150.0, 0.05, 500.0)
+SCENE_4096x3000_U16 = SceneSpec(4096, 3000, 64, 16, 0.0, 512.0, 1000.0, 60000.0, 150.0, 0.05, 500.0)
 
 BENCH_SEED = 0x600DF00D  # bench/cuda.cu:39 of the reference
 

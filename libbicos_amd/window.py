@@ -250,7 +250,7 @@ def train(spec=None, settings: Optional[TrainSettings] = None, opts: Optional[Wi
     (default: 1920x1200x33 u16) labelled by the plain path. Returns
     (params float32 array, history list of (step, mean loss))."""
     torch = _torch()
-    spec = spec or _scene.C1_1920x1200_U16
+    spec = spec or _scene.SCENE_1920x1200_U16
     st = settings or TrainSettings()
     opts = opts or WindowOptions()
     t0 = time.time()
@@ -315,7 +315,7 @@ def evaluate(params, spec=None, scenes: int = 200, opts: Optional[WindowOptions]
     paths (events around each call)."""
     torch = _torch()
     from . import Config as _Config
-    spec = spec or _scene.C1_1920x1200_U16
+    spec = spec or _scene.SCENE_1920x1200_U16
     opts = opts or WindowOptions()
     params = params_device(params, device)
     cfg_plain = _Config(**LABEL_CONFIG)

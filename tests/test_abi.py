@@ -123,16 +123,16 @@ def test_cpp_headers_compile():
 
 def test_scene_generator_is_deterministic():
     from libbicos_amd import scene
-    a = scene.generate(scene.C0_640x480_U8, seed=5, height=24, width=100, threads=1)
-    b = scene.generate(scene.C0_640x480_U8, seed=5, height=24, width=100, threads=4)
-    c = scene.generate(scene.C0_640x480_U8, seed=6, height=24, width=100)
+    a = scene.generate(scene.SCENE_640x480_U8, seed=5, height=24, width=100, threads=1)
+    b = scene.generate(scene.SCENE_640x480_U8, seed=5, height=24, width=100, threads=4)
+    c = scene.generate(scene.SCENE_640x480_U8, seed=6, height=24, width=100)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
     assert not np.array_equal(a[0], c[0])
     L, R, gt = a
     assert L.shape == (12, 24, 100) and L.dtype == np.uint8
     assert 8.0 <= gt.min() and gt.max() <= 72.0
-    u = scene.generate(scene.C1_1920x1200_U16, seed=5, height=8, width=300)[0]
+    u = scene.generate(scene.SCENE_1920x1200_U16, seed=5, height=8, width=300)[0]
     assert u.dtype == np.uint16 and u.max() > 40000
 
 

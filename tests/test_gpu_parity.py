@@ -306,10 +306,10 @@ def oracle_match(oracle, L, R, cfg, gpu):
                         double=cfg.precision == 1, consistency=cons, corrmap=True)
 
 
-def test_match_c0_640x480_u8_full(gpu, oracle):
+def test_match_640x480_u8_full(gpu, oracle):
     """640x480 u8, 12 frames, FULL, threshold 0.9, no subpixel, NoDuplicates."""
     from libbicos_amd import scene
-    L, R, gt = scene.generate(scene.C0_640x480_U8, seed=101)
+    L, R, gt = scene.generate(scene.SCENE_640x480_U8, seed=101)
     cfg = gpu.Config(nxcorr_threshold=0.9, mode=gpu.TransformMode.Full)
     got, got_cm = run_match(gpu, L, R, cfg)
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
@@ -319,10 +319,10 @@ def test_match_c0_640x480_u8_full(gpu, oracle):
     assert (np.abs(got[v] - gt[v]) <= 1).mean() > 0.95
 
 
-def test_match_c1_band_u16_limited_subpixel(gpu, oracle):
+def test_match_1920_band_u16_limited_subpixel(gpu, oracle):
     """1920 wide u16, 33 frames, LIMITED, min_variance 2, threshold 0.96, step 0.1."""
     from libbicos_amd import scene
-    L, R, gt = scene.generate(scene.C1_1920x1200_U16, seed=102, height=64)
+    L, R, gt = scene.generate(scene.SCENE_1920x1200_U16, seed=102, height=64)
     cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
     got, got_cm = run_match(gpu, L, R, cfg)
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
@@ -353,10 +353,10 @@ def test_match_integration_matrix(gpu, oracle, thr, variant):
         assert same(got_cm, exp_cm), step
 
 
-def test_match_c4_band_u16_64frames_consistency(gpu, oracle, search_impl):
+def test_match_4096_band_u16_64frames_consistency(gpu, oracle, search_impl):
     """4096 wide u16, 64 frames (256-bit), Consistency(max_lr_diff 1), step 0.05."""
     from libbicos_amd import scene
-    L, R, gt = scene.generate(scene.C4_4096x3000_U16, seed=104, height=6)
+    L, R, gt = scene.generate(scene.SCENE_4096x3000_U16, seed=104, height=6)
     cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.05, min_variance=2.0,
                      variant=gpu.Variant.Consistency(1, False))
     got, got_cm = run_match(gpu, L, R, cfg)
@@ -370,7 +370,7 @@ def test_match_c4_band_u16_64frames_consistency(gpu, oracle, search_impl):
 
 def test_match_precision_double_and_no_threshold(gpu, oracle):
     from libbicos_amd import scene
-    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=105, height=16, width=800, frames=20)
+    L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=105, height=16, width=800, frames=20)
     for cfg in (gpu.Config(nxcorr_threshold=0.8, subpixel_step=0.2, precision=gpu.Precision.Double),
                 gpu.Config(nxcorr_threshold=0.8, precision=gpu.Precision.Double, min_variance=5.0),
                 gpu.Config(nxcorr_threshold=None),
@@ -383,10 +383,10 @@ def test_match_precision_double_and_no_threshold(gpu, oracle):
             assert same(got_cm, exp_cm)
 
 
-def test_match_full_size_c1(gpu, oracle):
+def test_match_full_size_1920x1200x33(gpu, oracle):
     """The whole 1920x1200x33 u16 workload, bit-exact against the oracle."""
     from libbicos_amd import scene
-    L, R, gt = scene.generate(scene.C1_1920x1200_U16, seed=scene.BENCH_SEED)
+    L, R, gt = scene.generate(scene.SCENE_1920x1200_U16, seed=scene.BENCH_SEED)
     cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
     got, got_cm = run_match(gpu, L, R, cfg)
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
@@ -403,7 +403,7 @@ def test_workspace_reuse_across_sizes(gpu, oracle):
     from libbicos_amd import scene
     cfg = gpu.Config(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
     for rows, cols in ((40, 600), (8, 200), (60, 900), (8, 200)):
-        L, R, _ = scene.generate(scene.C0_640x480_U8, seed=rows * cols, height=rows, width=cols)
+        L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=rows * cols, height=rows, width=cols)
         got, got_cm = run_match(gpu, L, R, cfg)
         exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
         assert same(got, exp) and same(got_cm, exp_cm)
@@ -412,7 +412,7 @@ def test_workspace_reuse_across_sizes(gpu, oracle):
 def test_numpy_match_api(gpu, oracle):
     """pybicos-style host API: lists of numpy images in, numpy out."""
     from libbicos_amd import scene
-    L, R, _ = scene.generate(scene.C0_640x480_U8, seed=106, height=32, width=400)
+    L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=106, height=32, width=400)
     cfg = gpu.Config(nxcorr_threshold=0.9, mode=gpu.TransformMode.Full)
     d, cm = gpu.match(list(L), list(R), cfg)
     exp, exp_cm = oracle_match(oracle, L, R, cfg, gpu)
@@ -425,7 +425,7 @@ def test_cpp_api_cli(gpu, oracle, tmp_path):
     """BICOS::match through the C++ API (bicos_cli) with the reference CLI's
     option mapping: FULL unless --limited, min variance default 1.0."""
     from libbicos_amd import scene
-    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=107, height=20, width=640, frames=24)
+    L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=107, height=20, width=640, frames=24)
     (tmp_path / "l.raw").write_bytes(L.tobytes())
     (tmp_path / "r.raw").write_bytes(R.tobytes())
     cli = os.path.join(ROOT, "libbicos_amd", "bicos_cli")
@@ -449,7 +449,7 @@ def test_python_cli_png_stacks(gpu, oracle, tmp_path):
     points (src/cli.cpp:190-250)."""
     import pointcloud as PC
     from libbicos_amd import imageio, scene
-    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=111, height=24, width=500, frames=20)
+    L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=111, height=24, width=500, frames=20)
     for side, st in (("l", L), ("r", R)):
         (tmp_path / side).mkdir()
         for t in range(st.shape[0]):
@@ -492,7 +492,7 @@ def test_match_separately_allocated_frames(gpu, oracle):
     packed into a strided stack first; results are unchanged"""
     import torch
     from libbicos_amd import scene
-    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=109, height=24, width=700, frames=20)
+    L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=109, height=24, width=700, frames=20)
     cfg = gpu.Config(nxcorr_threshold=0.9, subpixel_step=0.25, min_variance=1.0)
     fl = [dev(L[t]).clone() for t in range(L.shape[0])]
     fr = [dev(R[t]).clone() for t in range(R.shape[0])][::-1][::-1]
@@ -522,7 +522,7 @@ def test_precision_half_mfma_vs_fp64_oracle(gpu, oracle):
     error of each result against the FP64 oracle (Precision::DOUBLE
     semantics, agree.cuh:35-65) is measured and bounded."""
     from libbicos_amd import scene
-    L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=110, height=96)
+    L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=110, height=96)
     cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
                      precision=gpu.Precision.HalfMFMA)
     got, got_cm = run_match(gpu, L, R, cfg)
@@ -542,19 +542,19 @@ def test_precision_half_mfma_vs_fp64_oracle(gpu, oracle):
     assert (np.isfinite(plain) | ~np.isfinite(got)).mean() > 0.99
 
 
-@pytest.mark.parametrize("which", ["c1", "c0"])
+@pytest.mark.parametrize("which", ["1920x1200_u16", "640x480_u8"])
 def test_precision_moments_vs_fp64_oracle(gpu, oracle, which):
     """MOMENTS subpixel mode: disparity error of each result against the FP64
     oracle (Precision::DOUBLE semantics, agree.cuh:35-65), measured and
     bounded; with a corrmap asked for, the exact FP32 stage runs and the
     result is bit-identical to it."""
     from libbicos_amd import scene
-    if which == "c1":
-        L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=112, height=96)
+    if which == "1920x1200_u16":
+        L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=112, height=96)
         kw = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
         okw = dict(thr=0.96, step=0.1, min_variance=2.0)
     else:
-        L, R, _ = scene.generate(scene.C0_640x480_U8, seed=113, height=64)
+        L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=113, height=64)
         kw = dict(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
         okw = dict(thr=0.9, step=0.25, full=True)
     cfg = gpu.Config(precision=gpu.Precision.Moments, **kw)
@@ -565,7 +565,7 @@ def test_precision_moments_vs_fp64_oracle(gpu, oracle, which):
     assert st["valid_both"] > 0.5 * got.size
     # one grid step where two grid points are within the rounding bound; the
     # 8-bit samples round coarsely, so their bound is wider
-    assert st["d_mean"] < 0.02 and st["d_over_0p1"] < (0.01 if which == "c1" else 0.08)
+    assert st["d_mean"] < 0.02 and st["d_over_0p1"] < (0.01 if which == "1920x1200_u16" else 0.08)
     assert st["d_max"] <= 2.0
     assert st["flip_frac"] == 0.0
     with_cm, cm = run_match(gpu, L, R, cfg)
@@ -573,18 +573,18 @@ def test_precision_moments_vs_fp64_oracle(gpu, oracle, which):
     assert same(with_cm, ex) and same(cm, ex_cm)
 
 
-@pytest.mark.parametrize("which", ["c1", "c0"])
+@pytest.mark.parametrize("which", ["1920x1200_u16", "640x480_u8"])
 def test_precision_moments_decisions_exact(gpu, oracle, which):
     """MOMENTS without a corrmap settles accept/reject by a bound on what the
     rounding can change and rescores only where the bound cannot decide:
     the valid set is exactly the exact FP32 path's, and the grid point
     matches it for nearly every pixel."""
     from libbicos_amd import scene
-    if which == "c1":
-        L, R, _ = scene.generate(scene.C1_1920x1200_U16, seed=114, height=96)
+    if which == "1920x1200_u16":
+        L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=114, height=96)
         kw = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
     else:
-        L, R, _ = scene.generate(scene.C0_640x480_U8, seed=115, height=64)
+        L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=115, height=64)
         kw = dict(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
     d = gpu.match_device(dev(L), dev(R), gpu.Config(precision=gpu.Precision.Moments, **kw))[0]
     got = host(d)
@@ -594,12 +594,12 @@ def test_precision_moments_decisions_exact(gpu, oracle, which):
     same = (got[v] == ex[v]).mean()
     print(f"MOMENTS {which} without corrmap: same grid point for {same:.4f} of valid pixels, "
           f"max |d| {np.abs(got[v] - ex[v]).max():.2f}")
-    assert same > (0.99 if which == "c1" else 0.9)
+    assert same > (0.99 if which == "1920x1200_u16" else 0.9)
 
 
 def test_precision_half_mfma_u8_full_mode(gpu, oracle):
     from libbicos_amd import scene
-    L, R, _ = scene.generate(scene.C0_640x480_U8, seed=111, height=64)
+    L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=111, height=64)
     cfg = gpu.Config(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full,
                      precision=gpu.Precision.HalfMFMA)
     got, got_cm = run_match(gpu, L, R, cfg)

@@ -297,7 +297,7 @@ def test_match_recovers_ground_truth(oracle):
     """On a clean synthetic scene the matched integer disparities sit within
     1 px of the generator's ground truth (640x480 u8, 12 frames, FULL, 0.9)."""
     from libbicos_amd import scene
-    L, R, gt = scene.generate(scene.C0_640x480_U8, seed=101, height=96)
+    L, R, gt = scene.generate(scene.SCENE_640x480_U8, seed=101, height=96)
     disp, _ = oracle.match(L, R, thr=0.9, full=True)
     v = disp != -32768
     assert v.mean() > 0.6

@@ -17,7 +17,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-C1_CFG = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+BENCH_CFG = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
 
 
 def host(t):
@@ -39,10 +39,10 @@ def W():
 
 # ---------------------------------------------------------------- scenes
 
-@pytest.mark.parametrize("which", ["c1", "c0"])
+@pytest.mark.parametrize("which", ["1920x1200_u16", "640x480_u8"])
 def test_scene_device_matches_host(W, which):
     from libbicos_amd import scene
-    spec = scene.C1_1920x1200_U16 if which == "c1" else scene.C0_640x480_U8
+    spec = scene.SCENE_1920x1200_U16 if which == "1920x1200_u16" else scene.SCENE_640x480_U8
     kw = dict(height=40, width=200, frames=5)
     L, R, D = scene.generate(spec, seed=11, threads=4, **kw)
     dL, dR, dD = W.scene_device(spec, 11, frames=kw["frames"], height=kw["height"], width=kw["width"])
@@ -57,7 +57,7 @@ def test_scene_device_matches_host(W, which):
 
 def test_scene_frames_independent_of_count(W):
     from libbicos_amd import scene
-    spec = scene.C1_1920x1200_U16
+    spec = scene.SCENE_1920x1200_U16
     a = W.scene_device(spec, 5, frames=6, height=32, width=96)
     b = W.scene_device(spec, 5, frames=4, height=32, width=96)
     assert np.array_equal(host(a[0])[:4], host(b[0]))
@@ -93,7 +93,7 @@ def test_labels(W, kind):
 
 def small_scene(W, seed, H=48, Wd=160):
     from libbicos_amd import scene
-    spec = scene.C1_1920x1200_U16
+    spec = scene.SCENE_1920x1200_U16
     L, R, D = W.scene_device(spec, seed, frames=4, height=H, width=Wd)
     Hq, Wq = H // 4, Wd // 4
     gt = host(D)[:Hq * 4, :Wq * 4].reshape(Hq, 4, Wq, 4).mean(axis=(1, 3)).astype(np.float32)
@@ -153,12 +153,12 @@ def test_adam_matches_formula(W):
 
 def test_training_reduces_loss(W):
     from libbicos_amd import scene
-    spec = scene.C1_1920x1200_U16
+    spec = scene.SCENE_1920x1200_U16
     import torch
     scenes = []
     for i in range(6):
         L, R, D = W.scene_device(spec, 900 + i, height=96, width=384)
-        d, _ = __import__("libbicos_amd").match_device(L, R, __import__("libbicos_amd").Config(**C1_CFG))
+        d, _ = __import__("libbicos_amd").match_device(L, R, __import__("libbicos_amd").Config(**BENCH_CFG))
         scenes.append((L[:4].contiguous(), R[:4].contiguous(), W.label(d)))
     params = W.params_device(W.init_params(0))
     m, v = torch.zeros_like(params), torch.zeros_like(params)
@@ -194,7 +194,7 @@ def test_mfma_network_matches_valu(W, bins):
     kernels' network (BF16 weights and activations, FP32 sums) up to
     summation order: same windows almost everywhere."""
     from libbicos_amd import scene
-    L, R, _ = W.scene_device(scene.C1_1920x1200_U16, 77, frames=4, height=96, width=640)
+    L, R, _ = W.scene_device(scene.SCENE_1920x1200_U16, 77, frames=4, height=96, width=640)
     p = W.load()[0] if os.path.exists(W.DEFAULT_WEIGHTS) else W.init_params(5)
     params = W.params_device(p)
     opts = W.WindowOptions(bins=bins)
@@ -217,7 +217,7 @@ def test_fused_network_equals_layers(W, bins):
     bit for bit (same products and rounding; the crop's 25 x 162 cells leave
     partial tiles at the right and bottom edges)."""
     from libbicos_amd import scene
-    L, R, _ = W.scene_device(scene.C1_1920x1200_U16, 78, frames=4, height=100, width=648)
+    L, R, _ = W.scene_device(scene.SCENE_1920x1200_U16, 78, frames=4, height=100, width=648)
     p = W.load()[0] if os.path.exists(W.DEFAULT_WEIGHTS) else W.init_params(6)
     params = W.params_device(p)
     opts = W.WindowOptions(bins=bins)
@@ -235,7 +235,7 @@ def test_fused_network_equals_layers(W, bins):
 
 def c1_crop(W, seed, H=64, Wd=480, frames=33):
     from libbicos_amd import scene
-    L, R, _ = W.scene_device(scene.C1_1920x1200_U16, seed, frames=frames, height=H, width=Wd)
+    L, R, _ = W.scene_device(scene.SCENE_1920x1200_U16, seed, frames=frames, height=H, width=Wd)
     return L, R
 
 
@@ -247,7 +247,7 @@ def test_full_width_window_equals_plain(W, variant, subpix, prec):
     the learned path must then reproduce the plain path bit for bit."""
     import libbicos_amd as B
     L, R = c1_crop(W, 31)
-    kw = dict(C1_CFG)
+    kw = dict(BENCH_CFG)
     if not subpix:
         kw.pop("subpixel_step")
     if prec == "moments":
@@ -291,7 +291,7 @@ def test_fallback_invariant_untrained(W, variant, subpix, hw, impl):
 
 def _fallback_invariant(W, B, variant, subpix, hw, moments):
     L, R = c1_crop(W, 32 + hw)
-    kw = dict(C1_CFG)
+    kw = dict(BENCH_CFG)
     if not subpix:
         kw.pop("subpixel_step")
     if moments:
@@ -347,7 +347,7 @@ def test_trained_windows_contain_labels(W):
     labelled cells whose window contains the plain-path disparity."""
     from libbicos_amd import scene
     params, _ = trained(W)
-    L, R, lab = W.scene_labels(scene.C1_1920x1200_U16, W.EVAL_SEED0 + 7)
+    L, R, lab = W.scene_labels(scene.SCENE_1920x1200_U16, W.EVAL_SEED0 + 7)
     c, hw, _ = W.predict(L, R, params)
     hit, err = W.window_hits(c, hw, lab)
     c0, hw0, _ = W.predict(L, R, W.params_device(W.init_params(0)))
@@ -356,16 +356,16 @@ def test_trained_windows_contain_labels(W):
     assert hit >= 0.97 and hit0 <= 0.5
 
 
-def test_trained_agreement_c1_full_size(W):
+def test_trained_agreement_full_size_1920x1200x33(W):
     """1920x1200x33 u16, unseen scenes: >= 99.5 % of the plain-valid pixels
     are valid in the learned path and within 0.5 px."""
     import libbicos_amd as B
     from libbicos_amd import scene
     params, _ = trained(W)
-    cfg = B.Config(precision=B.Precision.HalfMFMA, **C1_CFG)
+    cfg = B.Config(precision=B.Precision.HalfMFMA, **BENCH_CFG)
     for k in range(2):
-        L, R, _ = W.scene_device(scene.C1_1920x1200_U16, W.EVAL_SEED0 + 100 + k)
-        a = host(B.match_device(L, R, B.Config(**C1_CFG))[0])
+        L, R, _ = W.scene_device(scene.SCENE_1920x1200_U16, W.EVAL_SEED0 + 100 + k)
+        a = host(B.match_device(L, R, B.Config(**BENCH_CFG))[0])
         d, _, st = W.match_windowed_device(L, R, cfg, params, stats=True)
         b = host(d)
         va = np.isfinite(a)
