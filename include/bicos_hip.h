@@ -145,14 +145,15 @@ int bicos_hip_release_workspace(void);
 enum { BICOS_SEARCH_MFMA = 0, BICOS_SEARCH_VALU = 1 };
 int bicos_hip_set_search_impl(int32_t impl);
 
-/* Selects the kernel of the subpixel stage at Precision::SINGLE for
- * subsequent calls of this process: BICOS_SUBPIXEL_PRUNED (default; exact
- * moments and a rigorous closed-form bound pick the grid points that can be
- * the maximum, the reference's FP32 arithmetic scores only those) or
- * BICOS_SUBPIXEL_SCAN (the reference's FP32 arithmetic at every grid point).
- * Both give identical results. PRUNED applies without a corrmap and with at
- * most 64 grid points; SCAN is used otherwise. No reference counterpart
- * (include/impl/cpu/agree.hpp:94-185 scans every grid point). */
+/* Selects the kernel of the subpixel stage at Precision::SINGLE and
+ * Precision::DOUBLE for subsequent calls of this process:
+ * BICOS_SUBPIXEL_PRUNED (default; exact moments and a rigorous closed-form
+ * bound pick the grid points that can be the maximum, the reference's FP32
+ * or FP64 arithmetic scores only those) or BICOS_SUBPIXEL_SCAN (the
+ * reference's arithmetic at every grid point). Both give identical results.
+ * PRUNED applies without a corrmap and with at most 64 grid points; SCAN is
+ * used otherwise. No reference counterpart (include/impl/cpu/agree.hpp:94-185
+ * scans every grid point). */
 enum { BICOS_SUBPIXEL_PRUNED = 0, BICOS_SUBPIXEL_SCAN = 1 };
 int bicos_hip_set_subpixel_impl(int32_t impl);
 

@@ -437,9 +437,10 @@ SUBPIXEL_PRUNED, SUBPIXEL_SCAN = 0, 1
 
 
 def set_subpixel_impl(impl: int) -> None:
-    """Subpixel kernel at Precision.SINGLE for later calls: SUBPIXEL_PRUNED
-    (default; closed-form bounds pick the grid points the exact arithmetic
-    scores) or SUBPIXEL_SCAN (every grid point). Identical results."""
+    """Subpixel kernel at Precision.SINGLE / DOUBLE for later calls:
+    SUBPIXEL_PRUNED (default; closed-form bounds pick the grid points the
+    exact arithmetic scores) or SUBPIXEL_SCAN (every grid point). Identical
+    results."""
     _check(_lib.bicos_hip_set_subpixel_impl(int(impl)))
 
 

@@ -40,32 +40,34 @@ namespace bicos_hip {
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// The reference's nxcorr (agree.hpp:27-50) on the left pixel (col) and the
-// right samples it(t): means, then fma chains in frame order, streaming the
-// samples twice instead of holding them. Identical arithmetic to nxcorr().
-template<typename TIn, int NMAX, int N_EXACT, typename Right>
-__device__ __forceinline__ float nxcorr_stream(const Stack& f0, int row, int col, Right&& it,
+// The reference's nxcorr (agree.hpp:27-50; nxcorrd, agree.cuh:35-65, with
+// TPrec = double) on the left pixel (col) and the right samples it(t):
+// means, then fma chains in frame order, streaming the samples twice instead
+// of holding them. Identical arithmetic to nxcorr().
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT, typename Right>
+__device__ __forceinline__ TPrec nxcorr_stream(const Stack& f0, int row, int col, Right&& it,
                                                Depth<NMAX, N_EXACT> n, bool mv, float minvar) {
-    float mean0 = 0.0f, mean1 = 0.0f;
+    TPrec mean0 = 0, mean1 = 0;
 #pragma unroll 3
     for (int t = 0; t < NMAX; ++t)
         if (n.has(t)) {
-            mean0 += (float)load_sample_cached<TIn>(f0, t, row, col);
-            mean1 += it(t);
+            mean0 += (TPrec)load_sample_cached<TIn>(f0, t, row, col);
+            mean1 += (TPrec)it(t);
         }
-    mean0 /= (float)n.get();
-    mean1 /= (float)n.get();
-    float covar = 0.0f, var0 = 0.0f, var1 = 0.0f;
+    mean0 /= (TPrec)n.get();
+    mean1 /= (TPrec)n.get();
+    TPrec covar = 0, var0 = 0, var1 = 0;
 #pragma unroll 3
     for (int t = 0; t < NMAX; ++t)
         if (n.has(t)) {
-            const float d0 = (float)load_sample_cached<TIn>(f0, t, row, col) - mean0, d1 = it(t) - mean1;
-            covar = __builtin_fmaf(d0, d1, covar);
-            var0 = __builtin_fmaf(d0, d0, var0);
-            var1 = __builtin_fmaf(d1, d1, var1);
+            const TPrec d0 = (TPrec)load_sample_cached<TIn>(f0, t, row, col) - mean0,
+                        d1 = (TPrec)it(t) - mean1;
+            covar = pfma(d0, d1, covar);
+            var0 = pfma(d0, d0, var0);
+            var1 = pfma(d1, d1, var1);
         }
-    if (mv && (var0 < minvar || var1 < minvar))
-        return -1.0f;
+    if (mv && (var0 < (TPrec)minvar || var1 < (TPrec)minvar))
+        return (TPrec)-1;
     return covar / psqrt(var0 * var1);
 }
 
@@ -231,9 +233,9 @@ struct GridX {
 template<int NMAX>
 struct CandLds {
     static constexpr int CAND_PIXELS = NMAX <= 36 ? 16 : 8;
-    float4 xs[CAND_PIXELS * NMAX];  // (left sample - mean0, a, b, c) of frame t
+    float4 xs[CAND_PIXELS * NMAX];  // (left sample, a, b, c) of frame t
     int row[CAND_PIXELS], col[CAND_PIXELS], c1[CAND_PIXELS], pre[CAND_PIXELS];
-    float mean0[CAND_PIXELS];
+    double mean0[CAND_PIXELS];      // left mean in the correlation's precision
     uint64_t mask[CAND_PIXELS];
     int start[64];
 };
@@ -263,9 +265,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // inrange: every frame's parabola of the lane's pixel stays inside
 // [2, max(TIn) - 2], so the cast of agree.hpp:164 is the identity; a round
 // whose pixels all have it skips the cast.
-template<typename TIn, int NMAX, int N_EXACT>
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
 __device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int col, int d,
-                                      float mean0_own, bool inrange, const Stack& f0,
+                                      TPrec mean0_own, bool inrange, const Stack& f0,
                                       const Stack& f1, Depth<NMAX, N_EXACT> n, float thr,
                                       const GridX& gx, bool mv, float minvar, float& res,
                                       CandLds<NMAX>& L) {
@@ -297,7 +299,7 @@ __device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int co
             L.col[rank] = col;
             L.c1[rank] = col - d;
             L.pre[rank] = pre;
-            L.mean0[rank] = mean0_own;
+            L.mean0[rank] = (double)mean0_own;
             L.mask[rank] = cmask;
             L.start[pre] = 1;
         }
@@ -313,8 +315,7 @@ __device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int co
                 const float y0 = (float)load_sample_cached<TIn>(f1, t, r, c1 - 1);
                 const float y1 = (float)load_sample_cached<TIn>(f1, t, r, c1);
                 const float y2 = (float)load_sample_cached<TIn>(f1, t, r, c1 + 1);
-                L.xs[gg * NMAX + t] = make_float4(u - L.mean0[gg], 0.5f * (y0 - 2.0f * y1 + y2),
-                                                  0.5f * (-y0 + y2), y1);
+                L.xs[gg * NMAX + t] = make_float4(u, 0.5f * (y0 - 2.0f * y1 + y2), 0.5f * (-y0 + y2), y1);
             }
         }
         // this lane's pair: pixel p (rank among the takers), its j-th candidate
@@ -329,50 +330,53 @@ __device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int co
         const int k = on ? __builtin_ctzll(m) : 0;
         const float xe = gx.x[k];
         const float4* px = L.xs + p * NMAX;
+        const TPrec mean0 = (TPrec)L.mean0[p];
         const bool fast = __ballot(take && !inrange) == 0;  // uniform over the wave
-        // frame t's interpolated sample at xe (agree.hpp:164); c0 = left - mean0
+        // frame t's interpolated sample at xe (agree.hpp:164)
         const auto sample = [&](const float4& v) {
             const float r = __builtin_rintf(v.y * xe * xe + v.z * xe + v.w);
             return fast ? r : (float)(TIn)(int)r;
         };
         // both of the reference's passes recompute the samples (keeping them
-        // in registers would double the kernel's register count)
-        float mean1 = 0.0f, covar = 0.0f, var0 = 0.0f, var1 = 0.0f;
+        // in registers would double the kernel's register count); the
+        // correlation in TPrec as nxcorr / nxcorrd (agree.cuh:35-97)
+        TPrec mean1 = 0, covar = 0, var0 = 0, var1 = 0;
 #pragma unroll 8
         for (int t = 0; t < NMAX; ++t)
-            if (n.has(t)) mean1 += sample(px[t]);
-        mean1 /= (float)nf;
+            if (n.has(t)) mean1 += (TPrec)sample(px[t]);
+        mean1 /= (TPrec)nf;
 #pragma unroll 8
         for (int t = 0; t < NMAX; ++t)
             if (n.has(t)) {
                 const float4 v = px[t];
-                const float d1 = sample(v) - mean1;
-                var0 = __builtin_fmaf(v.x, v.x, var0);
-                covar = __builtin_fmaf(v.x, d1, covar);
-                var1 = __builtin_fmaf(d1, d1, var1);
+                const TPrec d0 = (TPrec)v.x - mean0, d1 = (TPrec)sample(v) - mean1;
+                var0 = pfma(d0, d0, var0);
+                covar = pfma(d0, d1, covar);
+                var1 = pfma(d1, d1, var1);
             }
-        float best = (mv && (var0 < minvar || var1 < minvar)) ? -1.0f : covar / psqrt(var0 * var1);
+        TPrec best = (mv && (var0 < (TPrec)minvar || var1 < (TPrec)minvar)) ? (TPrec)-1
+                                                                             : covar / psqrt(var0 * var1);
         int best_k = k;
-        if (!on || !(best > -1.0f)) {  // does not beat the initial -1 (NaN included)
-            best = -1.0f;
+        if (!on || !(best > (TPrec)-1)) {  // does not beat the initial -1 (NaN included)
+            best = (TPrec)-1;
             best_k = 0x7FFFFFFF;
         }
         // segmented reduction towards each pixel's first lane
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const float ob = __shfl_down(best, o);
+            const TPrec ob = __shfl_down(best, o);
             const int ok = __shfl_down(best_k, o);
             if (lane + o < seg_end && (ob > best || (ob == best && ok < best_k))) {
                 best = ob;
                 best_k = ok;
             }
         }
-        const float hb = __shfl(best, take ? pre : lane);
+        const TPrec hb = __shfl(best, take ? pre : lane);
         const int hk = __shfl(best_k, take ? pre : lane);
         if (take) {
             // the reference keeps x = 0 unless some point beats -1 (agree.hpp:159,168-171)
             const float bx = hk != 0x7FFFFFFF ? gx.x[hk] : 0.0f;
-            res = hb < thr ? NAN : (float)d - bx;
+            res = hb < (TPrec)thr ? NAN : (float)d - bx;
             need = false;
         }
         wave_lds_sync();  // the next round's staging must not overtake this round's reads
@@ -391,7 +395,7 @@ __device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int co
 // otherwise the exact candidate stage scores the candidates (all points for
 // an unsettled pixel). Output equals k_agree_subpixel's bit for bit; no
 // status is written. `lds` is the wave's CandLds (BX) or float4 slice.
-template<typename TIn, int NMAX, int N_EXACT, bool BX = false>
+template<typename TIn, int NMAX, int N_EXACT, bool BX = false, typename TPrec = float>
 __device__ __forceinline__ void agree_subpixel_moments_px(
     bool act, int row, int col, const int16_t* __restrict__ disp, int64_t dstep, const Stack& f0,
     const Stack& f1, int n_rt, int W, float thr, float step, int npts, int has_minvar, float minvar,
@@ -466,23 +470,23 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
         if (var0d < mvd - slack) {
             lowvar = true;
         } else if (var0d <= mvd + slack) {
-            float mean0 = 0.0f;
+            TPrec mean0 = 0;  // the reference's two-pass variance, in its precision
             for (int t = 0; t < NMAX; ++t)
-                if (n.has(t)) mean0 += (float)load_sample_cached<TIn>(f0, t, row, col);
-            mean0 /= (float)n.get();
-            float var0 = 0.0f;
+                if (n.has(t)) mean0 += (TPrec)load_sample_cached<TIn>(f0, t, row, col);
+            mean0 /= (TPrec)n.get();
+            TPrec var0 = 0;
             for (int t = 0; t < NMAX; ++t)
                 if (n.has(t)) {
-                    const float c = (float)load_sample_cached<TIn>(f0, t, row, col) - mean0;
-                    var0 = __builtin_fmaf(c, c, var0);
+                    const TPrec c = (TPrec)load_sample_cached<TIn>(f0, t, row, col) - mean0;
+                    var0 = pfma(c, c, var0);
                 }
-            lowvar = var0 < minvar;
+            lowvar = var0 < (TPrec)minvar;
         }
     }
     if (lowvar && thr > -1.0f) st |= ST_HOPELESS;
     if (valid_d) {
         if (!interior) {
-            const float corr = nxcorr_stream<TIn>(f0, row, col, [&](int t) {
+            const TPrec corr = nxcorr_stream<TIn, TPrec>(f0, row, col, [&](int t) {
                 return (float)load_sample_cached<TIn>(f1, t, row, col1);
             }, n, mv, minvar);
             if (!(corr < thr)) res = (float)d;
@@ -650,10 +654,12 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     const uint64_t pend = __ballot(act && (st & ST_PENDING));
     if constexpr (BX) {
         if (pend) {
-            const float mean0 = (float)su / (float)n.get();  // su: exact, below 2^24
+            // the reference's left mean in TPrec: su is the exact sample sum
+            const TPrec mean0 = (TPrec)su / (TPrec)n.get();
             const bool inrange = ylo >= 2 && yhi <= (int)(TIn)~(TIn)0 - 2;
-            exact_candidates_wave<TIn>(act && (st & ST_PENDING), cmask, row, col, d, mean0, inrange,
-                                       f0, f1, n, thr, *gx, mv, minvar, res, *(CandLds<NMAX>*)lds);
+            exact_candidates_wave<TIn, TPrec>(act && (st & ST_PENDING), cmask, row, col, d, mean0,
+                                              inrange, f0, f1, n, thr, *gx, mv, minvar, res,
+                                              *(CandLds<NMAX>*)lds);
         }
         if (act) out[(size_t)row * ostep + col] = res;
     } else {
@@ -681,9 +687,10 @@ __global__ __launch_bounds__(256) void k_agree_subpixel_moments(
                                                   ostep, status, xs[threadIdx.x >> 6]);
 }
 
-// Bit-exact subpixel agreement (Precision::SINGLE, no corrmap, npts <= 64):
+// Bit-exact subpixel agreement (Precision::SINGLE with TPrec = float,
+// Precision::DOUBLE with TPrec = double; no corrmap, npts <= 64):
 // closed-form pruning, the exact stage on the candidate points only.
-template<typename TIn, int NMAX, int N_EXACT>
+template<typename TIn, typename TPrec, int NMAX, int N_EXACT>
 __global__ __launch_bounds__(256) void k_agree_subpixel_pruned(
     const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1, int n_rt,
     int W, float thr, float step, int npts, int has_minvar, float minvar, float* __restrict__ out,
@@ -692,9 +699,10 @@ __global__ __launch_bounds__(256) void k_agree_subpixel_pruned(
     const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
     if ((col & ~63) >= W)  // whole wave past the row
         return;
-    agree_subpixel_moments_px<TIn, NMAX, N_EXACT, true>(col < W, row, min(col, W - 1), disp, dstep, f0, f1,
-                                                        n_rt, W, thr, step, npts, has_minvar, minvar, out,
-                                                        ostep, nullptr, &cl[threadIdx.x >> 6], &gx);
+    agree_subpixel_moments_px<TIn, NMAX, N_EXACT, true, TPrec>(col < W, row, min(col, W - 1), disp, dstep,
+                                                               f0, f1, n_rt, W, thr, step, npts, has_minvar,
+                                                               minvar, out, ostep, nullptr,
+                                                               &cl[threadIdx.x >> 6], &gx);
 }
 
 // over listed pixels only (dense numbering of the lists; each wave walks

@@ -408,7 +408,7 @@ int run_subpixel_half_t(const int16_t* disp, int64_t dstep, const Stack& f0,
 // Subpixel kernel at Precision::SINGLE: pruned (default) or full scan.
 int g_subpixel_impl = BICOS_SUBPIXEL_PRUNED;
 
-template<typename TIn>
+template<typename TIn, typename TPrec>
 int run_subpixel_pruned_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                           uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
                           float* out, int64_t ostep, hipStream_t s) {
@@ -421,7 +421,7 @@ int run_subpixel_pruned_t(const int16_t* disp, int64_t dstep, const Stack& f0, c
     }
     return with_nmax(n, [&](auto NM, auto NE) -> int {
         constexpr int NMAX = decltype(NM)::value, NEX = decltype(NE)::value;
-        LAUNCH("agree_subpixel_pruned", (k_agree_subpixel_pruned<TIn, NMAX, NEX>), dim3((W + 255) / 256, H), dim3(256), 0, s,
+        LAUNCH("agree_subpixel_pruned", (k_agree_subpixel_pruned<TIn, TPrec, NMAX, NEX>), dim3((W + 255) / 256, H), dim3(256), 0, s,
                disp, dstep, f0, f1, (int)n, W, thr, step, npts, has_mv, mv, out, ostep, gx);
         return BICOS_OK;
     });
@@ -512,12 +512,17 @@ int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stac
     }
     if (precision == BICOS_PRECISION_MOMENTS)  // a corrmap is written: the exact stage throughout
         precision = BICOS_PRECISION_SINGLE;
-    if (precision == BICOS_PRECISION_SINGLE && !corr && g_subpixel_impl == BICOS_SUBPIXEL_PRUNED) {
+    if ((precision == BICOS_PRECISION_SINGLE || precision == BICOS_PRECISION_DOUBLE) && !corr &&
+        g_subpixel_impl == BICOS_SUBPIXEL_PRUNED) {
         const int npts = subpixel_points(step);
         if (npts >= 1 && npts <= 64) {
+            if (precision == BICOS_PRECISION_DOUBLE)
+                return depth == BICOS_8U
+                    ? run_subpixel_pruned_t<uint8_t, double>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s)
+                    : run_subpixel_pruned_t<uint16_t, double>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s);
             return depth == BICOS_8U
-                ? run_subpixel_pruned_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s)
-                : run_subpixel_pruned_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s);
+                ? run_subpixel_pruned_t<uint8_t, float>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s)
+                : run_subpixel_pruned_t<uint16_t, float>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s);
         }
     }
     if (precision == BICOS_PRECISION_HALF_MFMA) {

@@ -230,13 +230,14 @@ def scan_subpixel(gpu):
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
-def test_agree_subpixel_pruned_exact(gpu, oracle, dtype):
-    """Precision::SINGLE without a corrmap runs the pruned kernel: bit-exact
+@pytest.mark.parametrize("double", [False, True])
+def test_agree_subpixel_pruned_exact(gpu, oracle, dtype, double):
+    """Precision::SINGLE / DOUBLE without a corrmap run the pruned kernel: bit-exact
     against the oracle on smooth rows (multi-candidate peaks), saturated rows
     (parabolas leaving the sample range: every grid point scored), constant
     and low-variance pixels, thresholds down to -1, and grid sizes 3..64
     (101 points fall back to the full scan)."""
-    rng = np.random.default_rng(3100 + (dtype == np.uint16))
+    rng = np.random.default_rng(3100 + (dtype == np.uint16) + 2 * double)
     hi = np.iinfo(dtype).max
     cases = ((10, 0.25, "smooth"), (33, 0.1, "smooth"), (12, 0.05, "smooth"), (64, 0.05, "smooth"),
              (5, 0.3, "smooth"), (33, 0.1, "saturated"), (20, 0.1, "flat"), (33, 0.02, "smooth"),
@@ -272,8 +273,9 @@ def test_agree_subpixel_pruned_exact(gpu, oracle, dtype):
         for thr in (float(rng.uniform(0.3, 0.95)), 0.999, -1.0):
             for minvar in (float(rng.uniform(0, 5)), None):
                 mv_n = None if minvar is None else np.float32(minvar) * np.float32(n)
-                exp, _ = oracle.agree_subpixel(disp, s0, s1, thr, step, mv_n)
-                got = host(gpu.agree_subpixel(dev(disp), dev(s0), dev(s1), thr, step, minvar, 0))
+                exp, _ = oracle.agree_subpixel(disp, s0, s1, thr, step, mv_n, double=double)
+                got = host(gpu.agree_subpixel(dev(disp), dev(s0), dev(s1), thr, step, minvar,
+                                              1 if double else 0))
                 bad = np.argwhere(~((got == exp) | (np.isnan(got) & np.isnan(exp))))
                 assert same(got, exp), (n, step, kind, thr, minvar, bad[:5])
 
@@ -381,6 +383,8 @@ def test_match_precision_double_and_no_threshold(gpu, oracle):
         assert (got_cm is None) == (exp_cm is None)
         if got_cm is not None:
             assert same(got_cm, exp_cm)
+        d, _ = gpu.match_device(dev(L), dev(R), cfg, want_corrmap=False)  # pruned when subpixel
+        assert same(host(d), exp)
 
 
 def test_match_full_size_1920x1200x33(gpu, oracle):
