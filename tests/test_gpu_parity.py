@@ -280,6 +280,63 @@ def test_agree_subpixel_pruned_exact(gpu, oracle, dtype, double):
                 assert same(got, exp), (n, step, kind, thr, minvar, bad[:5])
 
 
+def test_pruned_equals_scan_random_sweep(gpu):
+    """300 seeded random configurations (u8/u16, n 2..65, 8 grid steps,
+    thresholds -1..0.999, with and without min_variance, FP32 and FP64;
+    smooth, uniform, saturated, flat, low-contrast and scene inputs with
+    random, invalid and out-of-range disparities): the pruned kernel equals
+    the full-scan kernel (itself bit-exact against the oracle) bit for bit."""
+    from libbicos_amd import scene
+    rng = np.random.default_rng(4242)
+    try:
+        for case in range(300):
+            dtype = np.uint16 if rng.random() < 0.6 else np.uint8
+            hi = np.iinfo(dtype).max
+            n = int(rng.choice([2, 3, 4, 5, 8, 10, 12, 16, 20, 24, 33, 40, 48, 64, 65]))
+            step = float(rng.choice([0.5, 0.3, 0.25, 0.2, 0.1, 0.05, 0.0317, 0.035]))
+            rows, cols = int(rng.integers(4, 24)), int(rng.integers(64, 700))
+            kind = rng.choice(["smooth", "uniform", "saturated", "flat", "scene", "lowcontrast"])
+            if kind == "scene":
+                spec = scene.SCENE_1920x1200_U16 if dtype == np.uint16 else scene.SCENE_640x480_U8
+                s0, s1, _ = scene.generate(spec, int(rng.integers(1, 1 << 30)), frames=n, height=rows, width=cols)
+                disp = rng.integers(0, 40, size=(rows, cols)).astype(np.int16)
+            else:
+                x = np.arange(cols + 8, dtype=np.float64)
+                base = np.zeros((n, rows, cols + 8))
+                for t in range(n):
+                    ph, fr = rng.uniform(0, 6.3, (rows, 1, 3)), rng.uniform(0.05, 0.8, (rows, 1, 3))
+                    base[t] = np.sin(fr * x[None, :, None] + ph).sum(-1)
+                if kind == "uniform":
+                    base = rng.random(base.shape)
+                base = (base - base.min()) / max(base.max() - base.min(), 1e-9)
+                if kind == "saturated":
+                    base = np.clip(base * 1.8 - 0.4, 0, 1)
+                if kind == "lowcontrast":
+                    base = 0.5 + 0.002 * (base - 0.5)
+                base = np.rint(base * hi).astype(dtype)
+                if kind == "flat":
+                    base[:, :, ::5] = base[0:1, :, ::5]
+                sh = int(rng.integers(0, 4))
+                s0 = base[:, :, 4:cols + 4].copy()
+                noise = rng.normal(0, hi * float(rng.choice([0, 0.001, 0.01])), s0.shape)
+                s1 = np.clip(np.rint(base[:, :, 4 - sh:cols + 4 - sh] + noise), 0, hi).astype(dtype)
+                disp = np.full((rows, cols), sh, np.int16)
+            mix = rng.random((rows, cols))
+            disp[mix < 0.15] = rng.integers(-1, cols, size=int((mix < 0.15).sum()))
+            disp[mix > 0.95] = -32768
+            thr = float(rng.choice([-1.0, 0.0, 0.5, 0.9, 0.96, 0.99, 0.999]))
+            minvar = None if rng.random() < 0.3 else float(rng.choice([0.5, 2.0, 10.0, 100.0]))
+            prec = int(rng.integers(0, 2))
+            d_d, s0_d, s1_d = dev(disp), dev(s0), dev(s1)
+            gpu.set_subpixel_impl(gpu.SUBPIXEL_PRUNED)
+            a = host(gpu.agree_subpixel(d_d, s0_d, s1_d, thr, step, minvar, prec))
+            gpu.set_subpixel_impl(gpu.SUBPIXEL_SCAN)
+            b = host(gpu.agree_subpixel(d_d, s0_d, s1_d, thr, step, minvar, prec))
+            assert same(a, b), (case, dtype, n, step, kind, thr, minvar, prec)
+    finally:
+        gpu.set_subpixel_impl(gpu.SUBPIXEL_PRUNED)
+
+
 def test_agree_subpixel_scan_knob(gpu, oracle, scan_subpixel):
     """the full-scan kernel stays selectable and agrees with the oracle"""
     rng = np.random.default_rng(3200)
