@@ -36,9 +36,19 @@ namespace bicos_hip {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// 4 bits -> 4 bytes of 0/1 (bit b to byte b)
+// v_mul_u32_u24 (full rate; the 32-bit multiply is quarter rate). Written
+// out because the compiler folds a following shift into the constant, which
+// then no longer fits 24 bits and takes the 32-bit multiply.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// 4 bits -> 4 bytes of 0/1 (bit b to byte b): the nibble times 0x00204081
+// puts bit b at 8 b
 __device__ __forceinline__ uint32_t spread4(uint32_t x) {
-    return ((x & 0xFu) * 0x00204081u) & 0x01010101u;
+    return mul_u24(x & 0xFu, 0x00204081u) & 0x01010101u;
 }
 
 // 16 descriptor bits -> 16 int8 entries of 0/1
@@ -75,6 +85,22 @@ __device__ __forceinline__ v4i expand_pm64(uint32_t b16) {
 __device__ __forceinline__ int imed3(int a, int b, int c) {
     // lowered to v_med3_i32
     return max(min(a, b), min(max(a, b), c));
+}
+
+__device__ __forceinline__ int imin3(int a, int b, int c) {
+    // lowered to v_min3_i32
+    return min(min(a, b), c);
+}
+
+// The pair step of the no-duplicates reduction, t = med3(s, a, b) and
+// s' = min3(s, a, b), as two instructions: in C the two share a two-operand
+// min after the compiler's rewriting and take three. Inline asm gets no
+// MFMA-result hazard wait states from the compiler, so a and b must not be
+// the first reads of a matrix product's result: callers order this after a
+// compiler-visible read of the same product (through s).
+__device__ __forceinline__ void pair_step_asm(int& s, int a, int b, int& t) {
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(t) : "v"(s), "v"(a), "v"(b));
+    asm("v_min3_i32 %0, %1, %2, %3" : "=v"(s) : "v"(s), "v"(a), "v"(b));
 }
 
 // "no candidate" key: above every real key (|key| < 2^21 in both key forms)
@@ -145,7 +171,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
         s1[b] = NO_KEY;
         s2[b] = NO_KEY;
     }
-    const bool busy = !LIST || qbase < count;
+    // waves whose query tiles lie past the row (or the list) skip the
+    // arithmetic but take part in the staging
+    const bool busy = qbase < (LIST ? count : W);
 
     // candidate row of accumulator i within a tile (the C operand of the
     // first K step in the embedded form)
@@ -153,19 +181,43 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
 #pragma unroll
     for (int i = 0; i < 16; ++i) crow_init[i] = EMBED ? 8 * (i / 4) + 4 * g + (i % 4) : 0;
 
+    // Staging: a thread expands whole candidate columns (CPT per chunk). The
+    // raw words of the next chunk are loaded before the current chunk's
+    // tiles, so their latency hides behind the matrix-core work; columns past
+    // the row stage as zero bits.
+    constexpr int NT = WAVES * 64;
+    constexpr int CPT = (CH + NT - 1) / NT;
     const uint32_t* crow = dc + (size_t)row * W * WORDS;
+    uint32_t raw[CPT][WORDS];
+    auto fetch = [&](int c0) {
+        const int cn = min(CH, W - c0);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int m = (int)threadIdx.x + j * NT;
+#pragma unroll
+            for (int w = 0; w < WORDS; ++w)
+                raw[j][w] = m < cn ? crow[(size_t)(c0 + m) * WORDS + w] : 0u;
+        }
+    };
+    fetch(0);
     const int ntiles = (W + 31) / 32;
     for (int c0 = 0; c0 < W; c0 += CH) {
         const int cn = min(CH, W - c0);
         const int cnp = (cn + 31) & ~31;
         __syncthreads();
-        for (int it = threadIdx.x; it < cnp * 2 * KS; it += WAVES * 64) {
-            const int m = it / (2 * KS), c = it - m * (2 * KS);
-            uint32_t bits = 0;
-            if (m < cn) bits = (crow[(size_t)(c0 + m) * WORDS + (c >> 1)] >> (16 * (c & 1))) & 0xFFFFu;
-            lds[m * STRIDE + c] = EMBED ? expand0_64(bits) : expand01(bits);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int m = (int)threadIdx.x + j * NT;
+            if (m < cnp) {
+#pragma unroll
+                for (int c = 0; c < 2 * KS; ++c) {
+                    const uint32_t bits = (raw[j][c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+                    lds[m * STRIDE + c] = EMBED ? expand0_64(bits) : expand01(bits);
+                }
+            }
         }
         __syncthreads();
+        if (c0 + CH < W) fetch(c0 + CH);
 
         // full tiles; the NB product chains are independent, so matrix-core
         // work of one overlaps the VALU work of the previous
@@ -187,19 +239,42 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
+                // keys in pairs (x0, x1): with s1 <= s2 the two smallest so
+                // far, the new smallest is min3(s1, x0, x1) and the new
+                // second smallest min(s2, med3(s1, x0, x1)); the eight med3
+                // values of a tile fold into s2 with four min3. 1.25 VALU
+                // ops per output (0.5 without the duplicate test).
+                int t[8];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    // candidate row within the tile: 8 (i / 4) + 4 g + i % 4; the
-                    // tile base (and 4 g in the shifted form) live in the running
-                    // state offset
-                    const int key = EMBED ? acc[b][i]
-                                          : (int)(((uint32_t)acc[b][i] << 16) +
-                                                  ((256u << 16) + 8u * (i / 4) + (i % 4)));
-                    if constexpr (NODUPES) s2[b] = imed3(s1[b], s2[b], key);
-                    s1[b] = min(s1[b], key);
+                for (int p = 0; p < 8; ++p) {
+                    // candidate row within the tile of output i:
+                    // 8 (i / 4) + 4 g + i % 4; the tile base (and 4 g in the
+                    // shifted form) live in the running state offset
+                    int x[2];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int i = 2 * p + h;
+                        x[h] = EMBED ? acc[b][i]
+                                     : (int)(((uint32_t)acc[b][i] << 16) +
+                                             ((256u << 16) + 8u * (i / 4) + (i % 4)));
+                    }
+                    if constexpr (NODUPES) {
+                        // the first pair in C: its reads of the product get
+                        // the hazard wait states, and every later pair
+                        // depends on it through s1
+                        if (p == 0) {
+                            t[p] = imed3(s1[b], x[0], x[1]);
+                            s1[b] = imin3(s1[b], x[0], x[1]);
+                        } else {
+                            pair_step_asm(s1[b], x[0], x[1], t[p]);
+                        }
+                    } else {
+                        s1[b] = imin3(s1[b], x[0], x[1]);
+                    }
                 }
+                if constexpr (NODUPES)
+                    s2[b] = imin3(imin3(s2[b], t[0], t[1]), imin3(t[2], t[3], t[4]), imin3(t[5], t[6], t[7])) - 32;
                 s1[b] -= 32;
-                if constexpr (NODUPES) s2[b] -= 32;
             }
         }
         // ragged last tile of the row: candidates at or beyond W are masked
