@@ -29,6 +29,7 @@ METRIC = "matched Mpixel/s @1920x1200x33 u16; learned vs plain at >=99.5% px agr
 # MI355X published peaks
 HBM_PEAK_GBS = 8000.0
 INT8_DENSE_TOPS = 5000.0
+FP4_DENSE_TFLOPS = 10100.0
 VALU_SIMDS = 1024        # 256 CUs x 4 SIMDs; a wave64 VALU instruction takes 4 cycles
 CLOCK_HZ = 2.4e9
 
@@ -41,7 +42,7 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=30)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=0x600DF00D)
-    p.add_argument("--search", choices=("mfma", "valu"), default="mfma",
+    p.add_argument("--search", choices=("mfma", "mfma_i8", "valu"), default="mfma",
                    help="row-search kernel: int8 matrix cores (default) or vector ALU")
     p.add_argument("--workload", choices=("1920x1200x33", "4096x3000x64", "kernels"),
                    default="1920x1200x33",
@@ -52,19 +53,21 @@ def parse_args():
     return p.parse_args()
 
 
-def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int):
+def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int, fp4: bool = False):
     """Algorithmic work of one launch of `name` against the MI355X roof.
 
-    search: W*W*H candidates x (descriptor bits) AND-popcount pairs, counted
-    as int8 multiply-accumulates (2 ops each) against the dense int8 matrix
-    peak; transform / agree: bytes the algorithm must move (samples read,
-    outputs written) against HBM bandwidth."""
+    search: W*W*H candidates x (descriptor bits) multiply-accumulates (2 ops
+    each) against the dense matrix peak of the operand type the kernel used
+    (fp4: 10.1 PFLOP/s; int8: 5.0 POP/s); transform / agree: bytes the
+    algorithm must move (samples read, outputs written) against HBM
+    bandwidth."""
     s = avg_ms * 1e-3
     if name.startswith("search"):
         ops = 2.0 * W * W * H * 32 * words
-        return {"bound": "mfma", "achieved": ops / s / 1e12, "peak": INT8_DENSE_TOPS,
-                "unit": "TFLOP/s", "frac": ops / s / 1e12 / INT8_DENSE_TOPS,
-                "work_per_launch": ops, "work_unit": "int8-equivalent ops"}
+        peak = FP4_DENSE_TFLOPS if fp4 else INT8_DENSE_TOPS
+        return {"bound": "mfma", "achieved": ops / s / 1e12, "peak": peak,
+                "unit": "TFLOP/s", "frac": ops / s / 1e12 / peak,
+                "work_per_launch": ops, "work_unit": "fp4 MACs x 2" if fp4 else "int8 MACs x 2"}
     if name.startswith("agree_subpixel"):
         # bound by vector-ALU issue: instructions per wave (rocprofv3
         # SQ_INSTS_VALU / SQ_WAVES, committed) x waves x 4 cycles against
@@ -423,7 +426,10 @@ def main():
     cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
                    mode=B.TransformMode.Limited, variant=B.Variant.NoDuplicates())
     words = B.descriptor_bits(n, cfg.mode) // 32
-    B.set_search_impl(B.SEARCH_MFMA if args.search == "mfma" else B.SEARCH_VALU)
+    B.set_search_impl({"mfma": B.SEARCH_MFMA, "mfma_i8": B.SEARCH_MFMA_I8, "valu": B.SEARCH_VALU}[args.search])
+    # the default matrix-core search takes fp4 operands for 64- to 256-bit
+    # descriptors on rows up to 4096 columns
+    fp4 = args.search == "mfma" and words >= 2 and W <= 4096
 
     dL = torch.from_numpy(L.view(np.int16)).cuda().view(torch.uint16)
     dR = torch.from_numpy(R.view(np.int16)).cuda().view(torch.uint16)
@@ -455,11 +461,11 @@ def main():
     kernels = {k: {"launches": int(c), "avg_ms": t / c} for k, (c, t) in kstats.items()}
     per_step = {k: v["avg_ms"] * v["launches"] / args.steps for k, v in kernels.items()}
     dom = max(per_step, key=per_step.get)
-    roof = kernel_roofline(dom, kernels[dom]["avg_ms"], W, H, n, words)
+    roof = kernel_roofline(dom, kernels[dom]["avg_ms"], W, H, n, words, fp4)
     roof["kernel"] = dom
     roof["traffic"] = committed_traffic(dom)
     for k, v in kernels.items():
-        r = kernel_roofline(k, v["avg_ms"], W, H, n, words)
+        r = kernel_roofline(k, v["avg_ms"], W, H, n, words, fp4)
         v["bound"] = r["bound"]
         v["roofline_frac"] = round(r["frac"], 4)
 

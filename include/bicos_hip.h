@@ -137,12 +137,14 @@ const char* bicos_hip_last_error(void);
 int bicos_hip_release_workspace(void);
 
 /* Selects the row-search kernel for subsequent calls of this process:
- * BICOS_SEARCH_MFMA (default; Hamming costs from int8 matrix-core products)
- * or BICOS_SEARCH_VALU (xor/popcount on the vector ALU). Both give identical
- * results; the switch exists for testing and measurement. No reference
- * counterpart (the reference picks between its shared-memory and
+ * BICOS_SEARCH_MFMA (default; Hamming costs from matrix-core products: fp4
+ * operands for descriptors of 64 bits and more on rows up to 4096 columns,
+ * int8 otherwise), BICOS_SEARCH_MFMA_I8 (int8 matrix-core products
+ * throughout) or BICOS_SEARCH_VALU (xor/popcount on the vector ALU). All give
+ * identical results; the switch exists for testing and measurement. No
+ * reference counterpart (the reference picks between its shared-memory and
  * global-memory search kernels by row size, src/impl/cuda.cu.in:149-177). */
-enum { BICOS_SEARCH_MFMA = 0, BICOS_SEARCH_VALU = 1 };
+enum { BICOS_SEARCH_MFMA = 0, BICOS_SEARCH_VALU = 1, BICOS_SEARCH_MFMA_I8 = 2 };
 int bicos_hip_set_search_impl(int32_t impl);
 
 /* Selects the kernel of the subpixel stage at Precision::SINGLE and

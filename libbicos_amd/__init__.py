@@ -425,11 +425,13 @@ def timing_read() -> dict:
     return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(min(k, 64))}
 
 
-SEARCH_MFMA, SEARCH_VALU = 0, 1
+SEARCH_MFMA, SEARCH_VALU, SEARCH_MFMA_I8 = 0, 1, 2
 
 
 def set_search_impl(impl: int) -> None:
-    """Row-search kernel for later calls: SEARCH_MFMA (default) or SEARCH_VALU."""
+    """Row-search kernel for later calls: SEARCH_MFMA (default: fp4 matrix-core
+    products for descriptors of 64 bits and more on rows up to 4096 columns,
+    int8 otherwise), SEARCH_MFMA_I8 (int8 products throughout) or SEARCH_VALU."""
     _check(_lib.bicos_hip_set_search_impl(int(impl)))
 
 

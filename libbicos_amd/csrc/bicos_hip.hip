@@ -301,10 +301,40 @@ int run_search_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool
     return BICOS_OK;
 }
 
+template<int WORDS, int NB, int WAVES, int CH>
+int run_search_mfma4_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes,
+                       int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
+                       hipStream_t s) {
+    constexpr size_t lds = (size_t)CH * (WORDS + 1) * 16;
+    static_assert(lds <= 64 * 1024 + 512, "LDS chunk too large");
+    const dim3 grid((W + WAVES * NB * 32 - 1) / (WAVES * NB * 32), H), block(WAVES * 64);
+#define SM(ND, M) LAUNCH("search_mfma", (k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep, (const int16_t*)nullptr, (const int*)nullptr, (int16_t*)nullptr)
+    if (nodupes) {
+        if (mode == SEARCH_DISP) SM(true, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SM(true, SEARCH_REV);
+        else SM(true, SEARCH_LR);
+    } else {
+        if (mode == SEARCH_DISP) SM(false, SEARCH_DISP);
+        else if (mode == SEARCH_REV) SM(false, SEARCH_REV);
+        else SM(false, SEARCH_LR);
+    }
+#undef SM
+    return BICOS_OK;
+}
+
 int run_search(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
                int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
                hipStream_t s) {
-    if (g_search_impl == BICOS_SEARCH_MFMA) {
+    if (g_search_impl == BICOS_SEARCH_MFMA && words >= 2 && W <= EMBED_MAX_W) {
+        switch (words) {
+            // fp4 products (twice the int8 rate); chunks of about 64 KiB of LDS
+            case 2: return run_search_mfma4_w<2, 2, 8, 1344>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 4: return run_search_mfma4_w<4, 2, 8, 800>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 8: return run_search_mfma4_w<8, 2, 8, 448>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+        }
+    }
+    if (g_search_impl == BICOS_SEARCH_MFMA || g_search_impl == BICOS_SEARCH_MFMA_I8) {
         switch (words) {
             // 2 query tiles per wave, 8 waves per block, candidate chunks
             // filling 64 KiB of LDS (measured best of the tried shapes for
@@ -795,7 +825,7 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
 }
 
 int bicos_hip_set_search_impl(int32_t impl) {
-    if (impl != BICOS_SEARCH_MFMA && impl != BICOS_SEARCH_VALU)
+    if (impl != BICOS_SEARCH_MFMA && impl != BICOS_SEARCH_VALU && impl != BICOS_SEARCH_MFMA_I8)
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown search implementation");
     g_search_impl = impl;
     return BICOS_OK;

@@ -351,4 +351,237 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
     }
 }
 
+// ---------------------------------------------------------------------------
+// fp4 form (rows up to 4096 columns, descriptors of 64 bits and more).
+//
+// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands in fp4 (E2M1) runs at
+// twice the int8 rate and covers 64 descriptor bits per product. Candidate
+// bits become 1.0 / 0 (codes 0b0010 / 0), query bits +1.0 / -1.0 (codes
+// 0b0010 / 0b1010); the E8M0 block scales 2^6 and 2^6 make the product
+// 4096 * dot, with dot = popcount(c) - 2 popcount(c & q) as above. All values
+// are integers below 2^24, so the FP32 accumulation is exact. Lane l of an
+// operand holds row (column) l % 32; descriptor word 2 s + g of K step s
+// goes to lane half g, bit 8 r + j to nibble j of register r (the same
+// packing for both operands, which is all the product needs).
+//
+// Keys are FP32 values 2^20 + 4096 * dot + column (the accumulator starts
+// from 2^20 + the row within the tile), all positive, so their bit patterns
+// order like the values and the integer v_med3_i32 / v_min3_i32 reduction is
+// unchanged. The tile offset of the running state is an exact FP32
+// subtraction of 32 per tile.
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int KEY4_BASE = 1 << 20;
+constexpr int KEY4_NONE = 0x4B000000;   // bit pattern of 2^23: above every real key
+
+// 8 bits -> 8 nibbles, bit j at 4 j + SH: even and odd bits separately, so
+// that no two partial products of a 24-bit multiply meet in a kept bit.
+template<int SH>
+__device__ __forceinline__ uint32_t spread8_nib(uint32_t byte) {
+    const uint32_t e = mul_u24(byte & 0x55u, 0x41041u << SH) & (0x01010101u << SH);
+    const uint32_t o = mul_u24(byte & 0xAAu, 0x208208u << SH) & (0x10101010u << SH);
+    return e | o;
+}
+
+// 32 candidate bits -> 32 fp4 entries 1.0 / 0
+__device__ __forceinline__ uint4 expand_c_fp4(uint32_t w) {
+    return make_uint4(spread8_nib<1>(w & 0xFFu), spread8_nib<1>((w >> 8) & 0xFFu),
+                      spread8_nib<1>((w >> 16) & 0xFFu), spread8_nib<1>(w >> 24));
+}
+
+// 32 query bits -> 32 fp4 entries +1.0 (bit 0) / -1.0 (bit 1)
+__device__ __forceinline__ v8i expand_q_fp4(uint32_t w) {
+    v8i r = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r[k] = (int)((spread8_nib<0>((w >> (8 * k)) & 0xFFu) << 3) | 0x22222222u);
+    return r;
+}
+
+__device__ __forceinline__ int f_sub32(int key) {
+    return __float_as_int(__int_as_float(key) - 32.0f);
+}
+
+template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH, bool LIST = false>
+__global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
+    const uint32_t* __restrict__ dq, const uint32_t* __restrict__ dc, int W,
+    const int16_t* __restrict__ rev, int max_lr_diff, int16_t* __restrict__ out, int64_t ostep,
+    const int16_t* __restrict__ qcol, const int* __restrict__ row_count, int16_t* __restrict__ res) {
+    static_assert(WORDS % 2 == 0, "fp4 form: 64-bit K steps");
+    constexpr int KS = WORDS / 2;          // K steps of 64 bits
+    constexpr int STRIDE = WORDS + 1;      // uint4 per staged candidate (16 B pad)
+    constexpr int SCALE = 127 + 6;         // E8M0 2^6 for both operands
+    extern __shared__ uint4 lds[];
+
+    const int row = LIST ? blockIdx.x : blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nl = lane & 31, g = lane >> 5;
+    const int group = LIST ? blockIdx.y : blockIdx.x;
+    int count = 0;
+    if constexpr (LIST) {
+        count = row_count[row];
+        if (group * WAVES * NB * 32 >= count)
+            return;  // uniform over the block
+    }
+
+    v8i qop[NB][KS];
+    int s1[NB], s2[NB];
+    int qc[NB];
+    const int qbase = (group * WAVES + wave) * (NB * 32);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        int col = qbase + b * 32 + nl;
+        if constexpr (LIST) col = col < count ? qcol[(size_t)row * W + col] : -1;
+        qc[b] = col;
+        const uint32_t* src = dq + ((size_t)row * W + (col >= 0 && col < W ? col : 0)) * WORDS;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) qop[b][s] = expand_q_fp4(src[2 * s + g]);
+        s1[b] = KEY4_NONE;
+        s2[b] = KEY4_NONE;
+    }
+    const bool busy = qbase < (LIST ? count : W);
+
+    v16f crow_init;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) crow_init[i] = (float)(KEY4_BASE + 8 * (i / 4) + 4 * g + (i % 4));
+
+    constexpr int NT = WAVES * 64;
+    constexpr int CPT = (CH + NT - 1) / NT;
+    const uint32_t* crow = dc + (size_t)row * W * WORDS;
+    uint32_t raw[CPT][WORDS];
+    auto fetch = [&](int c0) {
+        const int cn = min(CH, W - c0);
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int m = (int)threadIdx.x + j * NT;
+#pragma unroll
+            for (int w = 0; w < WORDS; ++w)
+                raw[j][w] = m < cn ? crow[(size_t)(c0 + m) * WORDS + w] : 0u;
+        }
+    };
+    fetch(0);
+    const int ntiles = (W + 31) / 32;
+    for (int c0 = 0; c0 < W; c0 += CH) {
+        const int cn = min(CH, W - c0);
+        const int cnp = (cn + 31) & ~31;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int m = (int)threadIdx.x + j * NT;
+            if (m < cnp) {
+#pragma unroll
+                for (int w = 0; w < WORDS; ++w) lds[m * STRIDE + w] = expand_c_fp4(raw[j][w]);
+            }
+        }
+        __syncthreads();
+        if (c0 + CH < W) fetch(c0 + CH);
+
+        // full tiles, then the ragged last tile of the row (candidates at or
+        // beyond W masked)
+        const int nfull = busy ? cn / 32 : 0;
+        const int nt = busy ? cnp / 32 : 0;
+        for (int mt = 0; mt < nt; ++mt) {
+            v8i aop[KS];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const uint4 v = lds[(mt * 32 + nl) * STRIDE + 2 * s + g];
+                aop[s] = v8i{(int)v.x, (int)v.y, (int)v.z, (int)v.w, 0, 0, 0, 0};
+            }
+            v16f acc[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                acc[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aop[0], qop[b][0], crow_init, 4, 4,
+                                                                         0, SCALE, 0, SCALE);
+#pragma unroll
+                for (int s = 1; s < KS; ++s)
+                    acc[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aop[s], qop[b][s], acc[b], 4, 4,
+                                                                             0, SCALE, 0, SCALE);
+            }
+            if (mt < nfull) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    int t[8];
+#pragma unroll
+                    for (int p = 0; p < 8; ++p) {
+                        const int x0 = __float_as_int(acc[b][2 * p]), x1 = __float_as_int(acc[b][2 * p + 1]);
+                        if constexpr (NODUPES) {
+                            // the first pair in C: its reads of the product
+                            // get the hazard wait states, and every later
+                            // pair depends on it through s1
+                            if (p == 0) {
+                                t[p] = imed3(s1[b], x0, x1);
+                                s1[b] = imin3(s1[b], x0, x1);
+                            } else {
+                                pair_step_asm(s1[b], x0, x1, t[p]);
+                            }
+                        } else {
+                            s1[b] = imin3(s1[b], x0, x1);
+                        }
+                    }
+                    if constexpr (NODUPES)
+                        s2[b] = f_sub32(imin3(imin3(s2[b], t[0], t[1]), imin3(t[2], t[3], t[4]),
+                                              imin3(t[5], t[6], t[7])));
+                    s1[b] = f_sub32(s1[b]);
+                }
+            } else {
+                const int tbase = c0 + mt * 32;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        int key = __float_as_int(acc[b][i]);
+                        if (tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = KEY4_NONE;
+                        if constexpr (NODUPES) s2[b] = imed3(s1[b], s2[b], key);
+                        s1[b] = min(s1[b], key);
+                    }
+                    s1[b] = f_sub32(s1[b]);
+                    if constexpr (NODUPES) s2[b] = f_sub32(s2[b]);
+                }
+            }
+        }
+    }
+
+    // absolute keys 4096 * dot + column, then merge lanes l, l ^ 32
+    const float off = (float)(32 * ntiles - KEY4_BASE);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int b1 = (int)(__int_as_float(s1[b]) + off), b2 = (int)(__int_as_float(s2[b]) + off);
+        const int o1 = __shfl_xor(b1, 32);
+        const int o2 = __shfl_xor(b2, 32);
+        const int m1 = min(b1, o1);
+        const int m2 = min(max(b1, o1), min(b2, o2));
+        const int col = qc[b];
+        const bool valid_key = !(NODUPES && (m1 >> 12) == (m2 >> 12));
+        const int best = m1 & 4095;
+        if constexpr (LIST) {
+            const int i = qbase + b * 32 + nl;
+            if (g != 0 || i >= count)
+                continue;
+            const bool valid = col >= 0 && valid_key;
+            const int16_t bst = valid ? (int16_t)best : INVALID16;
+            if constexpr (MODE == SEARCH_REV)
+                res[(size_t)row * W + i] = bst;
+            else if (col >= 0)
+                out[(size_t)row * ostep + col] = valid ? (int16_t)(col - best) : INVALID16;
+            continue;
+        }
+        if (g != 0 || col >= W)
+            continue;
+        int16_t v = INVALID16;
+        if (valid_key) {
+            if constexpr (MODE == SEARCH_DISP) {
+                v = (int16_t)(col - best);
+            } else if constexpr (MODE == SEARCH_REV) {
+                v = (int16_t)best;
+            } else {
+                const int r0 = rev[(size_t)row * W + best];
+                if (r0 != INVALID16 && abs(col - r0) <= max_lr_diff)
+                    v = (int16_t)((col + r0) / 2 - best);
+            }
+        }
+        out[(size_t)row * ostep + col] = v;
+    }
+}
+
 }  // namespace bicos_hip

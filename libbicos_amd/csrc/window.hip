@@ -319,13 +319,37 @@ int run_search_list_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W,
     return BICOS_OK;
 }
 
+template<int WORDS, int NB, int WAVES, int CH>
+int run_search_list_mfma4_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes,
+                            int mode, const int16_t* qcol, const int* row_count, int16_t* res,
+                            int16_t* disp, int64_t dstep, hipStream_t s) {
+    constexpr size_t lds = (size_t)CH * (WORDS + 1) * 16;
+    const dim3 grid(H, (W + WAVES * NB * 32 - 1) / (WAVES * NB * 32)), block(WAVES * 64);
+#define SM(ND, M) LAUNCH("search_list_mfma", (k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH, true>), grid, block, lds, s, dq, dc, W, (const int16_t*)nullptr, 0, disp, dstep, qcol, row_count, res)
+    if (nodupes) {
+        if (mode == SEARCH_REV) SM(true, SEARCH_REV); else SM(true, SEARCH_DISP);
+    } else {
+        if (mode == SEARCH_REV) SM(false, SEARCH_REV); else SM(false, SEARCH_DISP);
+    }
+#undef SM
+    return BICOS_OK;
+}
+
 // Full-row search of the listed pixels: the full search's kernels with one
 // 32-query tile per wave (a row lists a tenth of its pixels or so, and
 // smaller query groups keep more of a workgroup's waves busy).
 int run_search_list(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
                     int mode, const int16_t* qcol, const int* row_count, int16_t* res,
                     int16_t* disp, int64_t dstep, hipStream_t s) {
-    if (g_search_impl == BICOS_SEARCH_MFMA) {
+    if (g_search_impl == BICOS_SEARCH_MFMA && words >= 2 && W <= EMBED_MAX_W) {
+        switch (words) {
+            case 2: return run_search_list_mfma4_w<2, 1, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 4: return run_search_list_mfma4_w<4, 1, 8, 800>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 8: return run_search_list_mfma4_w<8, 1, 8, 448>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+        }
+    }
+    if (g_search_impl == BICOS_SEARCH_MFMA || g_search_impl == BICOS_SEARCH_MFMA_I8) {
         switch (words) {
             case 1: return run_search_list_mfma_w<1, 1, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
             case 2: return run_search_list_mfma_w<2, 1, 8, 800>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);

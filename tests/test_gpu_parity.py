@@ -107,10 +107,13 @@ def test_transform_pitched_and_unaligned_frames(gpu, oracle, dtype):
 
 # --------------------------------------------------------------- search
 
-@pytest.fixture(params=["mfma", "valu"])
+@pytest.fixture(params=["mfma", "mfma_i8", "valu"])
 def search_impl(request, gpu):
-    """both row-search kernels: int8 matrix cores (default) and vector ALU"""
-    gpu.set_search_impl(gpu.SEARCH_MFMA if request.param == "mfma" else gpu.SEARCH_VALU)
+    """every row-search kernel: matrix cores (default: fp4 products for
+    descriptors of 64 bits and more, int8 for 32-bit ones), int8 matrix
+    cores throughout, and vector ALU"""
+    gpu.set_search_impl({"mfma": gpu.SEARCH_MFMA, "mfma_i8": gpu.SEARCH_MFMA_I8,
+                         "valu": gpu.SEARCH_VALU}[request.param])
     yield request.param
     gpu.set_search_impl(gpu.SEARCH_MFMA)
 

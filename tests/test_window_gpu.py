@@ -269,7 +269,7 @@ def test_full_width_window_equals_plain(W, variant, subpix, prec):
     assert np.array_equal(a, b, equal_nan=True)
 
 
-@pytest.mark.parametrize("impl", ["mfma", "valu", "moments"])
+@pytest.mark.parametrize("impl", ["mfma", "mfma_i8", "valu", "moments"])
 @pytest.mark.parametrize("variant", ["nodup", "cons"])
 @pytest.mark.parametrize("subpix", [True, False])
 @pytest.mark.parametrize("hw", [0, 3])
@@ -277,12 +277,12 @@ def test_fallback_invariant_untrained(W, variant, subpix, hw, impl):
     """With any network, every pixel the learned path rejects was sent
     through the exact full search and NCC stage: the plain path rejects it
     too. Pixels the learned path accepts either match the plain result or
-    pass the NCC threshold at another disparity. Both row-search kernels
-    (matrix cores, vector ALU) serve the listed pixels."""
+    pass the NCC threshold at another disparity. Every row-search kernel
+    (fp4 and int8 matrix cores, vector ALU) serves the listed pixels."""
     import libbicos_amd as B
     if impl == "moments" and not subpix:
         pytest.skip("the MOMENTS precision concerns the subpixel stage")
-    B.set_search_impl(B.SEARCH_VALU if impl == "valu" else B.SEARCH_MFMA)
+    B.set_search_impl({"valu": B.SEARCH_VALU, "mfma_i8": B.SEARCH_MFMA_I8}.get(impl, B.SEARCH_MFMA))
     try:
         _fallback_invariant(W, B, variant, subpix, hw, impl == "moments")
     finally:
