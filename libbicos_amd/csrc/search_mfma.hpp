@@ -12,10 +12,13 @@
 // popcount(q) is constant per query, so the matrix product ranks candidates
 // exactly like the Hamming cost. For rows up to 4096 columns the product is
 // scaled and offset so that it is the candidate's sort key itself (cost,
-// then column); each output then costs two VALU ops: second-smallest key
-// (v_med3_i32, no-duplicates only) and smallest key (v_min_i32). Keys are
-// unique per column, so the first strict minimum and the duplicate test come
-// out as in the reference scan, independent of evaluation order.
+// then column). The keys of a tile are reduced in pairs: smallest key with
+// v_min3_i32, second smallest (no-duplicates only) with v_med3_i32 per pair
+// and v_min3_i32 over the pairs, 1.25 VALU ops per output (0.5 without the
+// duplicate test). Keys are unique per column, so the first strict minimum
+// and the duplicate test come out as in the reference scan, independent of
+// evaluation order. The int8 kernel below serves 32-bit descriptors and rows
+// wider than 4096 columns; k_search_mfma_fp4 (further down) the rest.
 //
 // Layout. A block serves WAVES*NB*32 query pixels of one image row; each
 // wave holds NB query tiles of 32 pixels as MFMA B operands in registers
@@ -119,8 +122,8 @@ constexpr int EMBED_MAX_W = 4096;
 //   the key. Query entries are +64 / -64 and candidate entries 0 / 64, so the
 //   product is 4096 * dot; the accumulator of the first K step starts from the
 //   candidate's row within the 32-column tile instead of 0. The sum is
-//   4096 * dot + row, exact in int32, with no VALU work: each output then
-//   costs two VALU ops (v_med3_i32, v_min_i32) instead of three.
+//   4096 * dot + row, exact in int32, with no VALU work per output before
+//   the reduction.
 //
 // In both forms the running state is kept relative to the current tile
 // (minus 32 per tile), so the tile base never has to be added per output.
