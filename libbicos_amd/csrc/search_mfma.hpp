@@ -106,6 +106,16 @@ __device__ __forceinline__ void pair_step_asm(int& s, int a, int b, int& t) {
     asm("v_min3_i32 %0, %1, %2, %3" : "=v"(s) : "v"(s), "v"(a), "v"(b));
 }
 
+// The first pair of a tile: s' = min3(s, a, b) in C (the product's first
+// read, which gets the hazard wait states) and t = med3(s, a, b) written out
+// with s' as an extra, unused input, so that it cannot be scheduled before
+// that read.
+__device__ __forceinline__ void pair_step_first(int& s, int a, int b, int& t) {
+    const int s_new = imin3(s, a, b);
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(t) : "v"(s), "v"(a), "v"(b), "v"(s_new));
+    s = s_new;
+}
+
 // "no candidate" key: above every real key (|key| < 2^21 in both key forms)
 // and far enough below 2^31 that the state offsets (at most 32 * 1024 tiles)
 // cannot wrap it around.
@@ -483,15 +493,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
         // full tiles, then the ragged last tile of the row (candidates at or
         // beyond W masked)
         const int nfull = busy ? cn / 32 : 0;
-        const int nt = busy ? cnp / 32 : 0;
-        for (int mt = 0; mt < nt; ++mt) {
+        auto products = [&](int mt, v16f* acc) {
             v8i aop[KS];
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 const uint4 v = lds[(mt * 32 + nl) * STRIDE + 2 * s + g];
                 aop[s] = v8i{(int)v.x, (int)v.y, (int)v.z, (int)v.w, 0, 0, 0, 0};
             }
-            v16f acc[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 acc[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aop[0], qop[b][0], crow_init, 4, 4,
@@ -501,46 +509,48 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
                     acc[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aop[s], qop[b][s], acc[b], 4, 4,
                                                                              0, SCALE, 0, SCALE);
             }
-            if (mt < nfull) {
+        };
+        for (int mt = 0; mt < nfull; ++mt) {
+            v16f acc[NB];
+            products(mt, acc);
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    int t[8];
+            for (int b = 0; b < NB; ++b) {
+                int t[8];
 #pragma unroll
-                    for (int p = 0; p < 8; ++p) {
-                        const int x0 = __float_as_int(acc[b][2 * p]), x1 = __float_as_int(acc[b][2 * p + 1]);
-                        if constexpr (NODUPES) {
-                            // the first pair in C: its reads of the product
-                            // get the hazard wait states, and every later
-                            // pair depends on it through s1
-                            if (p == 0) {
-                                t[p] = imed3(s1[b], x0, x1);
-                                s1[b] = imin3(s1[b], x0, x1);
-                            } else {
-                                pair_step_asm(s1[b], x0, x1, t[p]);
-                            }
-                        } else {
-                            s1[b] = imin3(s1[b], x0, x1);
-                        }
+                for (int p = 0; p < 8; ++p) {
+                    const int x0 = __float_as_int(acc[b][2 * p]), x1 = __float_as_int(acc[b][2 * p + 1]);
+                    if constexpr (NODUPES) {
+                        // every pair after the first depends on the first
+                        // through s1
+                        if (p == 0)
+                            pair_step_first(s1[b], x0, x1, t[p]);
+                        else
+                            pair_step_asm(s1[b], x0, x1, t[p]);
+                    } else {
+                        s1[b] = imin3(s1[b], x0, x1);
                     }
-                    if constexpr (NODUPES)
-                        s2[b] = f_sub32(imin3(imin3(s2[b], t[0], t[1]), imin3(t[2], t[3], t[4]),
-                                              imin3(t[5], t[6], t[7])));
-                    s1[b] = f_sub32(s1[b]);
                 }
-            } else {
-                const int tbase = c0 + mt * 32;
+                if constexpr (NODUPES)
+                    s2[b] = f_sub32(imin3(imin3(s2[b], t[0], t[1]), imin3(t[2], t[3], t[4]),
+                                          imin3(t[5], t[6], t[7])));
+                s1[b] = f_sub32(s1[b]);
+            }
+        }
+        if (busy && nfull * 32 < cn) {
+            const int tbase = c0 + nfull * 32;
+            v16f acc[NB];
+            products(nfull, acc);
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
+            for (int b = 0; b < NB; ++b) {
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        int key = __float_as_int(acc[b][i]);
-                        if (tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = KEY4_NONE;
-                        if constexpr (NODUPES) s2[b] = imed3(s1[b], s2[b], key);
-                        s1[b] = min(s1[b], key);
-                    }
-                    s1[b] = f_sub32(s1[b]);
-                    if constexpr (NODUPES) s2[b] = f_sub32(s2[b]);
+                for (int i = 0; i < 16; ++i) {
+                    int key = __float_as_int(acc[b][i]);
+                    if (tbase + 8 * (i / 4) + 4 * g + (i % 4) >= W) key = KEY4_NONE;
+                    if constexpr (NODUPES) s2[b] = imed3(s1[b], s2[b], key);
+                    s1[b] = min(s1[b], key);
                 }
+                s1[b] = f_sub32(s1[b]);
+                if constexpr (NODUPES) s2[b] = f_sub32(s2[b]);
             }
         }
     }
