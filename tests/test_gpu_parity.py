@@ -153,6 +153,42 @@ def test_search_chunked_rows(gpu, oracle, search_impl):
                               oracle.search(d0, d1, flags, 2))
 
 
+def test_search_fp4_random_sweep(gpu):
+    """The fp4 matrix-core search (default for 64- to 256-bit descriptors)
+    against the vector-ALU and int8 kernels, which the tests above pin to the
+    oracle: 150 seeded cases over row widths around the tile (32) and chunk
+    boundaries (1344 / 800 / 448 columns) up to 4096, every flag set and L/R
+    limit, and random, duplicate-heavy and sparse descriptors."""
+    rng = np.random.default_rng(777)
+    edges = [1, 2, 31, 32, 33, 63, 447, 448, 449, 799, 800, 801, 1343, 1344, 1345, 1600, 1601, 4095, 4096]
+    try:
+        for case in range(150):
+            words = int(rng.choice([2, 4, 8]))
+            W = int(rng.choice(edges)) if rng.random() < 0.6 else int(rng.integers(1, 4097))
+            H = int(rng.integers(1, 4))
+            flags, lr = int(rng.integers(0, 4)), int(rng.integers(0, 5))
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                d0 = rng.integers(0, 2**32, size=(H, W, words), dtype=np.uint64).astype(np.uint32)
+                d1 = rng.integers(0, 2**32, size=(H, W, words), dtype=np.uint64).astype(np.uint32)
+            elif kind == 1:
+                pool = rng.integers(0, 2**32, size=(max(2, W // 8), words), dtype=np.uint64).astype(np.uint32)
+                d0 = pool[rng.integers(0, len(pool), size=(H, W))]
+                d1 = pool[rng.integers(0, len(pool), size=(H, W))]
+            else:
+                m = lambda: rng.integers(0, 2**32, size=(H, W, words), dtype=np.uint64).astype(np.uint32)
+                d0, d1 = m() & m() & m(), m() & m() & m()
+            q0, q1 = dev(d0), dev(d1)
+            got = {}
+            for name, impl in (("fp4", gpu.SEARCH_MFMA), ("i8", gpu.SEARCH_MFMA_I8), ("valu", gpu.SEARCH_VALU)):
+                gpu.set_search_impl(impl)
+                got[name] = host(gpu.search(q0, q1, flags, lr))
+            assert np.array_equal(got["fp4"], got["valu"]), (case, W, H, words, flags, lr, kind)
+            assert np.array_equal(got["i8"], got["valu"]), (case, W, H, words, flags, lr, kind)
+    finally:
+        gpu.set_search_impl(gpu.SEARCH_MFMA)
+
+
 def test_search_planted_duplicates(gpu, oracle, search_impl):
     cols = 700
     rng = np.random.default_rng(5)
