@@ -301,14 +301,31 @@ int run_search_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool
     return BICOS_OK;
 }
 
+// Dynamic LDS above the default 64 KiB per workgroup needs the function's
+// attribute raised once.
+int allow_dynamic_lds(const void* kernel, size_t lds) {
+    static std::mutex mu;
+    static std::map<const void*, size_t> raised;
+    if (lds <= 65536) return BICOS_OK;
+    std::lock_guard<std::mutex> lk(mu);
+    if (raised[kernel] >= lds) return BICOS_OK;
+    HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    raised[kernel] = lds;
+    return BICOS_OK;
+}
+
 template<int WORDS, int NB, int WAVES, int CH>
 int run_search_mfma4_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes,
                        int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
                        hipStream_t s) {
     constexpr size_t lds = (size_t)CH * (WORDS + 1) * 16;
-    static_assert(lds <= 64 * 1024 + 512, "LDS chunk too large");
+    static_assert(2 * lds <= 160 * 1024, "two blocks per CU must fit the LDS");
     const dim3 grid((W + WAVES * NB * 32 - 1) / (WAVES * NB * 32), H), block(WAVES * 64);
-#define SM(ND, M) LAUNCH("search_mfma", (k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep, (const int16_t*)nullptr, (const int*)nullptr, (int16_t*)nullptr)
+#define SM(ND, M) do { \
+        int rc_ = allow_dynamic_lds((const void*)k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH>, lds); \
+        if (rc_) return rc_; \
+        LAUNCH("search_mfma", (k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH>), grid, block, lds, s, dq, dc, W, rev, max_lr, out, ostep, (const int16_t*)nullptr, (const int*)nullptr, (int16_t*)nullptr); \
+    } while (0)
     if (nodupes) {
         if (mode == SEARCH_DISP) SM(true, SEARCH_DISP);
         else if (mode == SEARCH_REV) SM(true, SEARCH_REV);
@@ -327,9 +344,11 @@ int run_search(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, 
                hipStream_t s) {
     if (g_search_impl == BICOS_SEARCH_MFMA && words >= 2 && W <= EMBED_MAX_W) {
         switch (words) {
-            // fp4 products (twice the int8 rate); chunks of about 64 KiB of LDS
-            case 2: return run_search_mfma4_w<2, 2, 8, 1344>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 4: return run_search_mfma4_w<4, 2, 8, 800>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            // fp4 products (twice the int8 rate); chunks of up to 78 KiB of LDS
+            // (two blocks per CU); 256-bit descriptors keep 448 columns, where one
+            // prefetched column per thread keeps the kernel at four waves per SIMD
+            case 2: return run_search_mfma4_w<2, 2, 8, 1664>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 4: return run_search_mfma4_w<4, 2, 8, 960>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
             case 8: return run_search_mfma4_w<8, 2, 8, 448>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
