@@ -272,15 +272,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
                                              ((256u << 16) + 8u * (i / 4) + (i % 4)));
                     }
                     if constexpr (NODUPES) {
-                        // the first pair in C: its reads of the product get
-                        // the hazard wait states, and every later pair
-                        // depends on it through s1
-                        if (p == 0) {
-                            t[p] = imed3(s1[b], x[0], x[1]);
-                            s1[b] = imin3(s1[b], x[0], x[1]);
-                        } else {
+                        // every pair after the first depends on the first
+                        // through s1
+                        if (p == 0)
+                            pair_step_first(s1[b], x[0], x[1], t[p]);
+                        else
                             pair_step_asm(s1[b], x[0], x[1], t[p]);
-                        }
                     } else {
                         s1[b] = imin3(s1[b], x[0], x[1]);
                     }
