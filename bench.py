@@ -486,8 +486,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("u16 samples, fp4 Hamming products (exact f32 sums), f32 NCC" if fp4 else
-                  "u16 samples, int8 Hamming products (i32 sums), f32 NCC" if args.search == "mfma_i8" else
-                  "u16 samples, u32 Hamming, f32 NCC"),
+                  "u16 samples, u32 Hamming, f32 NCC" if args.search == "valu" else
+                  "u16 samples, int8 Hamming products (i32 sums), f32 NCC"),
         "data": "synthetic",
         "config": {"workload": f"plain BICOS::match {W}x{H}x{n} u16, LIMITED, min_variance 2.0, "
                                "nxcorr_threshold 0.96, subpixel_step 0.1, NoDuplicates",

@@ -147,6 +147,13 @@ int bicos_hip_release_workspace(void);
 enum { BICOS_SEARCH_MFMA = 0, BICOS_SEARCH_VALU = 1, BICOS_SEARCH_MFMA_I8 = 2 };
 int bicos_hip_set_search_impl(int32_t impl);
 
+/* Candidate columns per LDS chunk of the full-row search kernel that `impl`
+ * runs for descriptors of `words` u32 words on rows up to 4096 columns (the
+ * chunk and tile edges are where a row search changes shape; tests probe
+ * widths around them). 0 for BICOS_SEARCH_VALU, -1 for unknown arguments.
+ * No reference counterpart. */
+int bicos_hip_search_chunk_cols(int32_t impl, int32_t words);
+
 /* Selects the kernel of the subpixel stage at Precision::SINGLE and
  * Precision::DOUBLE for subsequent calls of this process:
  * BICOS_SUBPIXEL_PRUNED (default; exact moments and a rigorous closed-form
@@ -282,6 +289,22 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
                              const bicos_config* cfg, const float* params,
                              const bicos_window_options* opts, bicos_image* disparity,
                              bicos_image* corrmap, bicos_window_stats* stats, void* stream);
+
+/* The windowed Hamming search of bicos_hip_match_windowed as a stage (the
+ * learned path's counterpart of bicos_hip_search; no reference counterpart,
+ * the semantics are include/impl/cpu/bicos.hpp:50-113 restricted to a
+ * window). Pixel (row, col) lies in cell (min(row/4, rows/4 - 1),
+ * min(col/4, cols/4 - 1)) with centre c and half-width hw (device arrays of
+ * (rows/4) x (cols/4)); it searches the candidate columns
+ * [k - hw, k + hw] clipped to the row, k = col - rint(c) (half to even), an
+ * empty window giving -32768. flags: 1 = no duplicates (within the window),
+ * 2 = consistency (the matched column searches the left row back over
+ * [col - hw, col + hw]), 3 = both. Descriptor buffers and out as
+ * bicos_hip_search. */
+int bicos_hip_search_windowed(const uint32_t* desc0, const uint32_t* desc1, int32_t rows,
+                              int32_t cols, int32_t words, int32_t flags, int32_t max_lr_diff,
+                              const float* center, const int16_t* halfwidth, int16_t* out,
+                              void* stream);
 
 /* ---- synthetic scenes (training and test inputs; the host generator's
  *      construction evaluated on the device) ----

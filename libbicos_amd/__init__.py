@@ -89,6 +89,7 @@ def _load():
     L.bicos_hip_agree_subpixel.argtypes = [vp, P(_Image), P(_Image), u32, f32, f32, i32, f32,
                                            i32, vp, vp, vp]
     L.bicos_hip_set_search_impl.argtypes = [i32]
+    L.bicos_hip_search_chunk_cols.argtypes = [i32, i32]
     L.bicos_hip_set_subpixel_impl.argtypes = [i32]
     L.bicos_hip_timing_enable.argtypes = [i32]
     L.bicos_hip_timing_read.argtypes = [P(_KernelStat), i32]
@@ -107,6 +108,7 @@ def _load():
     L.bicos_hip_window_adam.argtypes = [vp, vp, vp, vp, vp, f32, f32, f32, f32, i32, vp]
     L.bicos_hip_window_predict.argtypes = [P(_Image), P(_Image), u32, vp, P(_WindowOptions), vp,
                                            vp, vp, vp]
+    L.bicos_hip_search_windowed.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp]
     L.bicos_hip_match_windowed.argtypes = [P(_Image), P(_Image), u32, P(_Config), vp,
                                            P(_WindowOptions), P(_Image), P(_Image),
                                            P(_WindowStats), vp]
@@ -123,14 +125,15 @@ _lib = _load()
 EXPORTED_SYMBOLS = (
     "bicos_hip_config_default", "bicos_hip_disparity_type", "bicos_hip_corrmap_type",
     "bicos_hip_descriptor_bits", "bicos_hip_match", "bicos_hip_last_error",
-    "bicos_hip_release_workspace", "bicos_hip_set_search_impl", "bicos_hip_set_subpixel_impl",
+    "bicos_hip_release_workspace", "bicos_hip_set_search_impl", "bicos_hip_search_chunk_cols",
+    "bicos_hip_set_subpixel_impl",
     "bicos_hip_transform", "bicos_hip_search", "bicos_hip_agree",
     "bicos_hip_agree_subpixel", "bicos_hip_timing_enable", "bicos_hip_timing_read",
     "bicos_hip_malloc", "bicos_hip_free", "bicos_hip_memcpy_h2d", "bicos_hip_memcpy_d2h",
     "bicos_hip_device_synchronize", "bicos_hip_window_param_count",
     "bicos_hip_window_options_default", "bicos_hip_window_init", "bicos_hip_window_label",
     "bicos_hip_window_loss_grad", "bicos_hip_window_adam", "bicos_hip_window_predict",
-    "bicos_hip_match_windowed", "bicos_hip_scene_generate", "bicos_hip_set_cnn_impl",
+    "bicos_hip_match_windowed", "bicos_hip_search_windowed", "bicos_hip_scene_generate", "bicos_hip_set_cnn_impl",
     "bicos_hip_reproject", "bicos_hip_pointcloud",
 )
 
@@ -433,6 +436,15 @@ def set_search_impl(impl: int) -> None:
     products for descriptors of 64 bits and more on rows up to 4096 columns,
     int8 otherwise), SEARCH_MFMA_I8 (int8 products throughout) or SEARCH_VALU."""
     _check(_lib.bicos_hip_set_search_impl(int(impl)))
+
+
+def search_chunk_cols(impl: int, words: int) -> int:
+    """Candidate columns per LDS chunk of the full-row search that `impl`
+    runs for `words`-word descriptors (0 for SEARCH_VALU)."""
+    k = _lib.bicos_hip_search_chunk_cols(int(impl), int(words))
+    if k < 0:
+        raise ValueError("unknown search implementation or descriptor width")
+    return k
 
 
 SUBPIXEL_PRUNED, SUBPIXEL_SCAN = 0, 1

@@ -33,8 +33,6 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["bicos_hip.hip", "window.hip", "scene_gpu.hip", "pointcloud.hip", "match.cpp"]
-HIP_DEPS = ["device_common.hpp", "host_common.hpp", "transform.hpp", "search.hpp", "search_mfma.hpp",
-            "agree.hpp", "agree_half.hpp", "cnn.hpp", "window.hpp", "scene_field.hpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -49,8 +49,11 @@ def parse_args(argv=None):
     p.add_argument("-t", "--threshold", type=float, default=0.75,
                    help="Minimum normalized cross correlation for a match to be accepted. "
                         "Set to 0.0 to disable.")
-    p.add_argument("-v", "--variance", type=float, default=1.0,
-                   help="Minimum intensity variance. Only active with --threshold.")
+    # shown as default 1.0, but (like cxxopts' count(), which does not count a
+    # default value) min_variance is only set when -v is given: None here
+    p.add_argument("-v", "--variance", type=float, default=None,
+                   help="Minimum intensity variance. Only active with --threshold. (default: 1.0, "
+                        "applied only when the option is given)")
     p.add_argument("-s", "--step", type=float, default=None,
                    help="Stepsize for subpixel interpolation. Only effective when threshold is set.")
     p.add_argument("-o", "--out", default="bicosdisp.png", help="Output file for disparity image.")
@@ -86,7 +89,7 @@ def config_from_args(args, B):
         cfg.subpixel_step = args.step
     if args.limited:
         cfg.mode = B.TransformMode.Limited
-    if args.variance is not None and args.variance > 0.0:
+    if args.variance is not None and args.variance > 0.0:  # args.count("variance"), cli.cpp:154-156
         cfg.min_variance = args.variance
     if args.double:
         cfg.precision = B.Precision.Double

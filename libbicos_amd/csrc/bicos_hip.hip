@@ -277,6 +277,15 @@ int run_search_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodu
 // Search implementation: matrix cores (default) or vector ALU.
 int g_search_impl = BICOS_SEARCH_MFMA;
 
+// Candidate columns per LDS chunk of the full-row matrix-core searches,
+// indexed by descriptor words (1, 2, 4, 8; 0 = no such kernel). fp4: up to
+// 78 KiB of LDS (two blocks per CU); 256-bit descriptors keep 448 columns,
+// where one prefetched column per thread keeps the kernel at four waves per
+// SIMD. int8: chunks filling 64 KiB (measured best of the tried shapes for
+// 1920-wide rows of 128-bit descriptors).
+constexpr int FP4_CHUNK_COLS[9] = {0, 0, 1664, 0, 960, 0, 0, 0, 448};
+constexpr int I8_CHUNK_COLS[9] = {0, 1344, 800, 0, 448, 0, 0, 0, 224};
+
 template<int WORDS, int NB, int WAVES, int CH>
 int run_search_mfma_w(const uint32_t* dq, const uint32_t* dc, int H, int W, bool nodupes,
                       int mode, const int16_t* rev, int max_lr, int16_t* out, int64_t ostep,
@@ -344,24 +353,20 @@ int run_search(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, 
                hipStream_t s) {
     if (g_search_impl == BICOS_SEARCH_MFMA && words >= 2 && W <= EMBED_MAX_W) {
         switch (words) {
-            // fp4 products (twice the int8 rate); chunks of up to 78 KiB of LDS
-            // (two blocks per CU); 256-bit descriptors keep 448 columns, where one
-            // prefetched column per thread keeps the kernel at four waves per SIMD
-            case 2: return run_search_mfma4_w<2, 2, 8, 1664>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 4: return run_search_mfma4_w<4, 2, 8, 960>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 8: return run_search_mfma4_w<8, 2, 8, 448>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            // fp4 products (twice the int8 rate)
+            case 2: return run_search_mfma4_w<2, 2, 8, FP4_CHUNK_COLS[2]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 4: return run_search_mfma4_w<4, 2, 8, FP4_CHUNK_COLS[4]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 8: return run_search_mfma4_w<8, 2, 8, FP4_CHUNK_COLS[8]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
     }
     if (g_search_impl == BICOS_SEARCH_MFMA || g_search_impl == BICOS_SEARCH_MFMA_I8) {
         switch (words) {
-            // 2 query tiles per wave, 8 waves per block, candidate chunks
-            // filling 64 KiB of LDS (measured best of the tried shapes for
-            // 1920-wide rows of 128-bit descriptors)
-            case 1: return run_search_mfma_w<1, 2, 8, 1344>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 2: return run_search_mfma_w<2, 2, 8, 800>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 4: return run_search_mfma_w<4, 2, 8, 448>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
-            case 8: return run_search_mfma_w<8, 2, 8, 224>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            // 2 query tiles per wave, 8 waves per block
+            case 1: return run_search_mfma_w<1, 2, 8, I8_CHUNK_COLS[1]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 2: return run_search_mfma_w<2, 2, 8, I8_CHUNK_COLS[2]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 4: return run_search_mfma_w<4, 2, 8, I8_CHUNK_COLS[4]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
+            case 8: return run_search_mfma_w<8, 2, 8, I8_CHUNK_COLS[8]>(dq, dc, H, W, nodupes, mode, rev, max_lr, out, ostep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
     }
@@ -848,6 +853,13 @@ int bicos_hip_set_search_impl(int32_t impl) {
         return fail(BICOS_ERR_INVALID_ARGUMENT, "unknown search implementation");
     g_search_impl = impl;
     return BICOS_OK;
+}
+
+int bicos_hip_search_chunk_cols(int32_t impl, int32_t words) {
+    if (words != 1 && words != 2 && words != 4 && words != 8) return -1;
+    if (impl == BICOS_SEARCH_MFMA && words >= 2) return FP4_CHUNK_COLS[words];
+    if (impl == BICOS_SEARCH_MFMA || impl == BICOS_SEARCH_MFMA_I8) return I8_CHUNK_COLS[words];
+    return impl == BICOS_SEARCH_VALU ? 0 : -1;
 }
 
 int bicos_hip_set_subpixel_impl(int32_t impl) {

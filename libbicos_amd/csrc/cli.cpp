@@ -41,6 +41,7 @@ int main(int argc, char** argv) {
     std::string left, right, out = "bicosdisp.raw", corr;
     int n = 0, rows = 0, cols = 0, depth = 8;
     float thr = 0.75f, var = 1.0f;
+    bool has_var = false;  // cxxopts count(): set only when -v is given
     bool has_step = false, limited = false, dbl = false, no_dupes = false, has_lr = false;
     float step = 0.0f;
     int lr = 0;
@@ -59,7 +60,7 @@ int main(int argc, char** argv) {
         else if (a == "--cols") cols = std::stoi(next());
         else if (a == "--depth") depth = std::stoi(next());
         else if (a == "-t" || a == "--threshold") thr = std::stof(next());
-        else if (a == "-v" || a == "--variance") var = std::stof(next());
+        else if (a == "-v" || a == "--variance") { has_var = true; var = std::stof(next()); }
         else if (a == "-s" || a == "--step") { has_step = true; step = std::stof(next()); }
         else if (a == "-m" || a == "--lr-maxdiff") { has_lr = true; lr = std::stoi(next()); }
         else if (a == "--limited") limited = true;
@@ -79,7 +80,7 @@ int main(int argc, char** argv) {
     if (need_corrmap && !c.nxcorr_threshold.has_value()) c.nxcorr_threshold = -1.0f;
     if (has_step) c.subpixel_step = step;
     if (limited) c.mode = TransformMode::LIMITED;
-    if (var > 0.0f) c.min_variance = var;
+    if (has_var && var > 0.0f) c.min_variance = var;
     if (dbl) c.precision = Precision::DOUBLE;
     if (has_lr) c.variant = Variant::Consistency{lr, no_dupes};
 

@@ -557,6 +557,23 @@ int bicos_hip_window_predict(const bicos_image* stack0, const bicos_image* stack
     return rc ? rc : rc2;
 }
 
+int bicos_hip_search_windowed(const uint32_t* desc0, const uint32_t* desc1, int32_t rows,
+                              int32_t cols, int32_t words, int32_t flags, int32_t max_lr_diff,
+                              const float* center, const int16_t* halfwidth, int16_t* out,
+                              void* stream) {
+    if (!desc0 || !desc1 || !center || !halfwidth || !out)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "null buffer");
+    if (rows < CNN_DS || cols < CNN_DS)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "the learned window needs at least 4 rows and columns");
+    if (rows > 65535 || cols > 32767) return fail(BICOS_ERR_INVALID_ARGUMENT, "image too large");
+    if (flags < 0 || flags > 3) return fail(BICOS_ERR_INVALID_ARGUMENT, "flags must be 0..3");
+    Grid g;
+    g.H = rows; g.W = cols; g.Hq = rows / CNN_DS; g.Wq = cols / CNN_DS; g.K = 0;
+    g.cells = (size_t)g.Hq * g.Wq;
+    return run_window_search(desc0, desc1, g, words, (flags & 1) != 0, (flags & 2) != 0,
+                             max_lr_diff, center, halfwidth, out, cols, (hipStream_t)stream);
+}
+
 int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
                              const bicos_config* cfg, const float* params,
                              const bicos_window_options* opts, bicos_image* disparity,

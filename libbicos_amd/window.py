@@ -128,6 +128,24 @@ def predict(stack0, stack1, params, opts: Optional[WindowOptions] = None, stream
     return center, hw, peak
 
 
+def search_windowed(desc0, desc1, flags: int, max_lr_diff: int, center, halfwidth, out=None,
+                    stream=None):
+    """The windowed row search alone: (rows, cols, words) descriptor tensors
+    and (rows // 4, cols // 4) cell windows -> int16 disparity."""
+    torch = _torch()
+    rows, cols, words = desc0.shape
+    if out is None:
+        out = torch.empty((rows, cols), dtype=torch.int16, device=desc0.device)
+    if tuple(center.shape) != (rows // CELL, cols // CELL) or tuple(halfwidth.shape) != tuple(center.shape):
+        raise ValueError("center/halfwidth must be (rows // 4, cols // 4)")
+    if center.dtype != torch.float32 or halfwidth.dtype != torch.int16:
+        raise ValueError("center must be float32 and halfwidth int16")
+    _check(_lib.bicos_hip_search_windowed(_ptr(desc0), _ptr(desc1), rows, cols, words, int(flags),
+                                          int(max_lr_diff), _ptr(center.contiguous()),
+                                          _ptr(halfwidth.contiguous()), _ptr(out), _stream_ptr(stream)))
+    return out
+
+
 def label(disparity, min_valid: int = 8, stream=None):
     """Cell labels (rows // 4, cols // 4) from a plain-path disparity map."""
     torch = _torch()

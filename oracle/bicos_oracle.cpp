@@ -337,6 +337,103 @@ static void bicos_dispatch(const TDesc* d0, const TDesc* d1, int flags, int max_
     }
 }
 
+/* ---------------------------------------------------------- windowed search
+ *
+ * The learned-window path's Hamming search (no reference counterpart): the
+ * reference's bicos_search / bicos (include/impl/cpu/bicos.hpp:50-113) with
+ * the candidate scan limited to a window. Pixel (row, col) belongs to the
+ * 4 x 4 cell (min(row / 4, Hq - 1), min(col / 4, Wq - 1)), Hq = rows / 4,
+ * Wq = cols / 4; the cell gives a centre disparity c (float) and a half-width
+ * hw. The candidates are col1 = lo..hi with
+ *   k  = col - rint(c)            (round half to even)
+ *   lo = max(k - hw, 0),  hi = min(k + hw, cols - 1);
+ * an empty window (lo > hi) gives INVALID. The first strict minimum and the
+ * no-duplicates rule are bicos.hpp:59-73's, over the window only. With
+ * CONSISTENCY the matched right column searches the left row back over
+ * [max(col - hw, 0), min(col + hw, cols - 1)] (same flags), and the LR check
+ * and midpoint are bicos.hpp:99-106's. */
+struct CellWindows {
+    const float* center;
+    const int16_t* halfwidth;
+    int Hq, Wq;
+    int cell(int row, int col) const {
+        return std::min(row / 4, Hq - 1) * Wq + std::min(col / 4, Wq - 1);
+    }
+};
+
+/* include/impl/cpu/bicos.hpp:50-76, scanning col1 = lo..hi */
+template<typename TDesc, int FLAGS>
+static int bicos_search_window(const TDesc& d0, const TDesc* row1, int lo, int hi) {
+    int best_col1 = INVALID_DISP<int>, min_cost = std::numeric_limits<int>::max(),
+        num_duplicate_minima = 0;
+
+    for (int col1 = lo; col1 <= hi; ++col1) {
+        int cost = ham(d0, row1[col1]);
+
+        if (cost < min_cost) {
+            min_cost = cost;
+            best_col1 = col1;
+            if constexpr (FLAGS & FLAG_NODUPES)
+                num_duplicate_minima = 0;
+        } else if constexpr (FLAGS & FLAG_NODUPES) {
+            if (cost == min_cost)
+                num_duplicate_minima++;
+        }
+    }
+
+    if constexpr (FLAGS & FLAG_NODUPES)
+        if (0 < num_duplicate_minima)
+            return INVALID_DISP<int>;
+
+    return best_col1;
+}
+
+/* include/impl/cpu/bicos.hpp:78-113 with the windows above */
+template<typename TDesc, int FLAGS>
+static void bicos_window(const TDesc* desc0, const TDesc* desc1, int max_lr_diff, int rows,
+                         int cols, const CellWindows& win, int16_t* out) {
+    std::fill(out, out + (size_t)rows * cols, INVALID_DISP<int16_t>);
+
+    parallel_rows(rows, [&](int row) {
+        const TDesc *drow0 = desc0 + (size_t)row * cols, *drow1 = desc1 + (size_t)row * cols;
+        int16_t* orow = out + (size_t)row * cols;
+
+        for (int col0 = 0; col0 < cols; ++col0) {
+            const int q = win.cell(row, col0);
+            const int k = col0 - (int)std::nearbyint(win.center[q]);
+            const int hw = win.halfwidth[q];
+            const int lo = std::max(k - hw, 0), hi = std::min(k + hw, cols - 1);
+
+            int best_col1 = bicos_search_window<TDesc, FLAGS>(drow0[col0], drow1, lo, hi);
+
+            if (is_invalid(best_col1))
+                continue;
+
+            if constexpr (FLAGS & FLAG_CONSISTENCY) {
+                int reverse_col0 = bicos_search_window<TDesc, FLAGS>(
+                    drow1[best_col1], drow0, std::max(col0 - hw, 0), std::min(col0 + hw, cols - 1));
+
+                if (is_invalid(reverse_col0) || std::abs(col0 - reverse_col0) > max_lr_diff)
+                    continue;
+
+                orow[col0] = (int16_t)((col0 + reverse_col0) / 2 - best_col1);
+            } else
+                orow[col0] = (int16_t)(col0 - best_col1);
+        }
+    });
+}
+
+template<typename TDesc>
+static void bicos_window_dispatch(const TDesc* d0, const TDesc* d1, int flags, int max_lr_diff,
+                                  int rows, int cols, const CellWindows& win, int16_t* out) {
+    switch (flags) {
+        case 0: bicos_window<TDesc, 0>(d0, d1, max_lr_diff, rows, cols, win, out); break;
+        case 1: bicos_window<TDesc, 1>(d0, d1, max_lr_diff, rows, cols, win, out); break;
+        case 2: bicos_window<TDesc, 2>(d0, d1, max_lr_diff, rows, cols, win, out); break;
+        default: bicos_window<TDesc, 3>(d0, d1, max_lr_diff, rows, cols, win, out); break;
+    }
+}
+
 /* ------------------------------------------------------------------- agree */
 
 /* include/impl/cpu/agree.hpp:27-50 (FP32). Compiled without FP contraction:
@@ -608,6 +705,118 @@ static void match_impl(const void* const* f0, const void* const* f1, int64_t ste
     }
 }
 
+/* The left-side variance as the NCC stage forms it: sequential mean, then
+ * the fma chain of the centred samples (agree.hpp:31-44 in FP32,
+ * agree.cuh:35-56 in FP64). */
+template<typename TPrec, typename TInput>
+static TPrec left_variance(const TInput* p0, size_t n) {
+    if constexpr (std::is_same_v<TPrec, float>) {
+        float mean0 = 0.f;
+        for (size_t i = 0; i < n; ++i) mean0 += p0[i];
+        mean0 /= n;
+        float var0 = 0.f;
+        for (size_t i = 0; i < n; ++i) {
+            float diff0 = p0[i] - mean0;
+            var0 = std::fmaf(diff0, diff0, var0);
+        }
+        return var0;
+    } else {
+        double mean0 = 0.0;
+        for (size_t i = 0; i < n; ++i) mean0 = mean0 + (double)p0[i];
+        mean0 = mean0 / (double)n;
+        double var0 = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            double diff0 = p0[i] - mean0;
+            var0 = std::fma(diff0, diff0, var0);
+        }
+        return var0;
+    }
+}
+
+/* BICOS::match of the learned-window path (no reference counterpart):
+ * match_impl (src/impl/cpu.cpp:37-94) with bicos() replaced by bicos_window()
+ * and, when a threshold is set and `fallback` is 1 (all) or 2 (unsure), a
+ * second pass for the fallback set: the pixels the NCC stage left invalid,
+ * except (a) those whose left variance is below minvar when a minimum
+ * variance and a threshold above -1 are set (the NCC stage rejects every
+ * match of such a pixel: nxcorr returns -1, agree.hpp:46-47) and (b) with
+ * fallback 2, those whose cell has the half-width hw_lo (the confident
+ * cells). A fallback pixel gets the full-row search of bicos()
+ * (bicos.hpp:78-113, the plain path's) and the NCC stage again; its output
+ * and corrmap entry are those of that second pass. Every other pixel keeps
+ * the windowed pass's. *n_fallback receives the size of the set. */
+template<typename TInput, typename TDesc, typename TPrec>
+static void match_windowed_impl(const void* const* f0, const void* const* f1, int64_t step, int n,
+                                int rows, int cols, const Config& cfg, float minvar_n,
+                                const CellWindows& win, int hw_lo, int fallback, int16_t* disp16,
+                                float* dispf, TPrec* corrmap, int64_t* n_fallback) {
+    std::vector<TDesc> desc0, desc1;
+    descriptor_transform<TInput, TDesc>(f0, step, n, rows, cols, cfg.full, desc0);
+    descriptor_transform<TInput, TDesc>(f1, step, n, rows, cols, cfg.full, desc1);
+
+    int flags = FLAG_NODUPES, lr = -1;
+    if (cfg.consistency) {
+        flags = FLAG_CONSISTENCY | (cfg.no_dupes ? FLAG_NODUPES : 0);
+        lr = cfg.max_lr_diff;
+    }
+    bicos_window_dispatch<TDesc>(desc0.data(), desc1.data(), flags, lr, rows, cols, win, disp16);
+    *n_fallback = 0;
+    if (!cfg.has_thr)
+        return;
+
+    const size_t np = (size_t)rows * cols;
+    const TPrec nan = std::numeric_limits<TPrec>::quiet_NaN();
+    if (corrmap)
+        std::fill(corrmap, corrmap + np, nan);
+    auto stage = [&](int16_t* d, float* out, TPrec* cm) {
+        if (cfg.has_step)
+            agree_subpixel<TInput, TPrec>(d, f0, f1, step, n, rows, cols, cfg.thr, cfg.step,
+                                          cfg.has_minvar, minvar_n, out, cm);
+        else
+            agree<TInput, TPrec>(d, f0, f1, step, n, rows, cols, cfg.thr, cfg.has_minvar, minvar_n,
+                                 cm);
+    };
+    stage(disp16, dispf, corrmap);
+    if (fallback != 1 && fallback != 2)
+        return;
+
+    std::vector<uint8_t> take(np, 0);
+    int64_t count = 0;
+    TInput p0[128];
+    for (int row = 0; row < rows; ++row)
+        for (int col = 0; col < cols; ++col) {
+            const size_t i = (size_t)row * cols + col;
+            bool t = cfg.has_step ? std::isnan(dispf[i]) : is_invalid(disp16[i]);
+            if (t && fallback == 2 && win.halfwidth[win.cell(row, col)] == hw_lo)
+                t = false;
+            if (t && cfg.has_minvar && cfg.thr > -1.f) {
+                gather<TInput>(f0, step, n, row, col, p0);
+                if (left_variance<TPrec>(p0, (size_t)n) < (TPrec)minvar_n)
+                    t = false;
+            }
+            take[i] = t;
+            count += t;
+        }
+    *n_fallback = count;
+
+    // the full-row search (the plain path's) for the fallback set
+    std::vector<int16_t> full(np);
+    bicos_dispatch<TDesc>(desc0.data(), desc1.data(), flags, lr, rows, cols, full.data());
+    for (size_t i = 0; i < np; ++i)
+        if (!take[i]) full[i] = INVALID_DISP<int16_t>;
+    std::vector<float> outf(cfg.has_step ? np : 0);
+    std::vector<TPrec> cm2(corrmap ? np : 0, nan);
+    stage(full.data(), outf.data(), corrmap ? cm2.data() : nullptr);
+    for (size_t i = 0; i < np; ++i)
+        if (take[i]) {
+            if (cfg.has_step)
+                dispf[i] = outf[i];
+            else
+                disp16[i] = full[i];
+            if (corrmap) corrmap[i] = cm2[i];
+        }
+}
+
 } // namespace oracle
 
 using namespace oracle;
@@ -748,6 +957,136 @@ int oracle_match(const void* const* f0, const void* const* f1, int64_t step, int
         }
     }
 #undef OM
+    *is_float = (cfg->has_thr && cfg->has_step) ? 1 : 0;
+    return 0;
+}
+
+/* One interpolated subpixel sample as agree.hpp:152-164 forms it: the
+ * parabola coefficients of (y0, y1, y2), evaluated at x, rounded and cast to
+ * the input type (cast_interp). depth 1: u8, 2: u16. */
+int oracle_interp_sample(int depth, int y0, int y1, int y2, float x) {
+    const float fy0 = (float)y0, fy1 = (float)y1, fy2 = (float)y2;
+    const float a = 0.5f * (fy0 - 2.0f * fy1 + fy2);
+    const float b = 0.5f * (-fy0 + fy2);
+    const float c = fy1;
+    const float v = a * x * x + b * x + c;
+    return depth == 1 ? (int)cast_interp<uint8_t>(v) : (int)cast_interp<uint16_t>(v);
+}
+
+/* The FP32 subpixel correlation curve of agree.hpp:143-171 for a batch of
+ * pixels, without min_variance: for pixel i with left samples
+ * left[i * n + t] and right columns right3[(i * 3 + j) * n + t] (j = 0, 1, 2
+ * for col1 - 1, col1, col1 + 1), out[i * npts + k] = nxcorr(left,
+ * interp(x_k)) at the grid points x_k of agree.hpp:162 (npts of them). */
+int oracle_subpixel_curve(int depth, int n, int64_t npix, const void* left, const void* right3,
+                          float step, int npts, float* out) {
+    if (!(step > 0.f) || n < 2 || n > 128) { g_err = "bad arguments"; return -1; }
+    auto run = [&](auto tag) {
+        using TInput = decltype(tag);
+        const TInput* L = (const TInput*)left;
+        const TInput* R = (const TInput*)right3;
+        const int64_t chunk = 4096, nchunk = (npix + chunk - 1) / chunk;
+        parallel_rows((int)nchunk, [&](int ci) {
+            TInput interp[128];
+            float a[128], b[128], c[128];
+            for (int64_t i = ci * chunk; i < std::min(npix, (ci + 1) * chunk); ++i) {
+                const TInput* p0 = L + i * n;
+                const TInput *y0 = R + (i * 3) * n, *y1 = y0 + n, *y2 = y1 + n;
+                for (int t = 0; t < n; ++t) {
+                    a[t] = 0.5f * (y0[t] - 2.0f * y1[t] + y2[t]);
+                    b[t] = 0.5f * (-y0[t] + y2[t]);
+                    c[t] = y1[t];
+                }
+                int k = 0;
+                for (float x = -1.f; x <= 1.f && k < npts; x += step, ++k) {
+                    for (int t = 0; t < n; ++t)
+                        interp[t] = cast_interp<TInput>(a[t] * x * x + b[t] * x + c[t]);
+                    out[i * npts + k] = nxcorr(p0, interp, (size_t)n, false, 0.f);
+                }
+            }
+        });
+    };
+    if (depth == 1) run(uint8_t{});
+    else if (depth == 2) run(uint16_t{});
+    else { g_err = "bad depth"; return -1; }
+    return 0;
+}
+
+/* Window-restricted row search (bicos_window above) on descriptor words.
+ * center / halfwidth: (rows / 4) x (cols / 4) cell arrays. */
+int oracle_search_window(const uint32_t* d0, const uint32_t* d1, int rows, int cols, int words,
+                         int flags, int max_lr_diff, const float* center, const int16_t* halfwidth,
+                         int16_t* out) {
+    if (rows < 4 || cols < 4) { g_err = "windows need at least 4 rows and columns"; return -1; }
+    const CellWindows win{center, halfwidth, rows / 4, cols / 4};
+    const int w = words;
+    auto run = [&](auto tag) {
+        using TDesc = decltype(tag);
+        std::vector<TDesc> a((size_t)rows * cols), b((size_t)rows * cols);
+        for (size_t i = 0; i < a.size(); ++i) {
+            a[i] = load_words<TDesc>(d0 + i * w);
+            b[i] = load_words<TDesc>(d1 + i * w);
+        }
+        bicos_window_dispatch<TDesc>(a.data(), b.data(), flags, max_lr_diff, rows, cols, win, out);
+    };
+    switch (words) {
+        case 1: run(uint32_t{}); break;
+        case 2: run(uint64_t{}); break;
+        case 4: run(uint128_t{}); break;
+        case 8: run(std::bitset<256>{}); break;
+        default: g_err = "bad words"; return -1;
+    }
+    return 0;
+}
+
+/* BICOS::match of the learned-window path (match_windowed_impl above) on
+ * the CPU. Arguments as oracle_match, plus the cell windows (center,
+ * halfwidth: (rows / 4) x (cols / 4)), hw_lo (the confident cells'
+ * half-width), fallback (0 none, 1 all, 2 unsure) and *n_fallback. */
+int oracle_match_windowed(const void* const* f0, const void* const* f1, int64_t step, int depth,
+                          int n, int rows, int cols, const oracle::Config* cfg, const float* center,
+                          const int16_t* halfwidth, int hw_lo, int fallback, int16_t* disp16,
+                          float* dispf, int* is_float, void* corrmap, int64_t* n_fallback) {
+    if (n < 2) { g_err = "need at least two images"; return -1; }
+    if (depth != 1 && depth != 2) { g_err = "bad input depths"; return -1; }
+    if (rows < 4 || cols < 4) { g_err = "windows need at least 4 rows and columns"; return -1; }
+    if (cfg->has_thr && cfg->has_step && !(cfg->step > 0.f)) { g_err = "subpixel_step must be positive"; return -1; }
+    const int bits = required_bits(n, cfg->full != 0);
+    const int words = words_for_bits(bits);
+    if (bits < 0 || words == 0) {
+        g_err = "input stacks too large, would require " + std::to_string(bits) + " bits";
+        return -1;
+    }
+    const float minvar_n = cfg->has_minvar ? cfg->minvar * (float)n : 0.f;
+    const CellWindows win{center, halfwidth, rows / 4, cols / 4};
+
+#define OW(TI, TD)                                                                                  \
+    do {                                                                                            \
+        if (cfg->dbl)                                                                               \
+            match_windowed_impl<TI, TD, double>(f0, f1, step, n, rows, cols, *cfg, minvar_n, win,   \
+                                                hw_lo, fallback, disp16, dispf, (double*)corrmap,   \
+                                                n_fallback);                                        \
+        else                                                                                        \
+            match_windowed_impl<TI, TD, float>(f0, f1, step, n, rows, cols, *cfg, minvar_n, win,    \
+                                               hw_lo, fallback, disp16, dispf, (float*)corrmap,     \
+                                               n_fallback);                                         \
+    } while (0)
+    if (depth == 1) {
+        switch (words) {
+            case 1: OW(uint8_t, uint32_t); break;
+            case 2: OW(uint8_t, uint64_t); break;
+            case 4: OW(uint8_t, uint128_t); break;
+            default: OW(uint8_t, std::bitset<256>); break;
+        }
+    } else {
+        switch (words) {
+            case 1: OW(uint16_t, uint32_t); break;
+            case 2: OW(uint16_t, uint64_t); break;
+            case 4: OW(uint16_t, uint128_t); break;
+            default: OW(uint16_t, std::bitset<256>); break;
+        }
+    }
+#undef OW
     *is_float = (cfg->has_thr && cfg->has_step) ? 1 : 0;
     return 0;
 }

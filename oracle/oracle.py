@@ -62,6 +62,12 @@ def lib():
         L.oracle_subpixel_grid.argtypes = [f32, vp, i32]
         L.oracle_match.argtypes = [vp, vp, i64, i32, i32, i32, i32, C.POINTER(OracleConfig), vp,
                                    vp, C.POINTER(C.c_int), vp]
+        L.oracle_search_window.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp, vp]
+        L.oracle_match_windowed.argtypes = [vp, vp, i64, i32, i32, i32, i32, C.POINTER(OracleConfig),
+                                            vp, vp, i32, i32, vp, vp, C.POINTER(C.c_int), vp,
+                                            C.POINTER(C.c_int64)]
+        L.oracle_interp_sample.argtypes = [i32, i32, i32, i32, f32]
+        L.oracle_subpixel_curve.argtypes = [i32, i32, i64, vp, vp, f32, i32, vp]
         L.oracle_required_bits.argtypes = [i32, i32]
         L.oracle_descriptor_words.argtypes = [i32, i32]
         L.oracle_set_threads.argtypes = [i32]
@@ -176,6 +182,27 @@ def subpixel_grid(step: float) -> np.ndarray:
     return buf[:k].copy()
 
 
+def interp_sample(y0: int, y1: int, y2: int, x: float, dtype=np.uint16) -> int:
+    """One interpolated subpixel sample (agree.hpp:152-164), cast to dtype."""
+    return lib().oracle_interp_sample(1 if np.dtype(dtype) == np.uint8 else 2, int(y0), int(y1),
+                                      int(y2), float(x))
+
+
+def subpixel_curve(left: np.ndarray, right3: np.ndarray, step: float) -> np.ndarray:
+    """FP32 subpixel correlation of every grid point (agree.hpp:143-171, no
+    min_variance) for a batch of pixels: left (P, n), right3 (P, 3, n) of
+    uint8/uint16 -> (P, npts) float32."""
+    left = np.ascontiguousarray(left)
+    right3 = np.ascontiguousarray(right3, left.dtype)
+    P, n = left.shape
+    assert right3.shape == (P, 3, n)
+    npts = len(subpixel_grid(step))
+    out = np.empty((P, npts), np.float32)
+    _check(lib().oracle_subpixel_curve(left.dtype.itemsize, n, P, left.ctypes.data,
+                                       right3.ctypes.data, float(step), npts, out.ctypes.data))
+    return out
+
+
 def match(s0: np.ndarray, s1: np.ndarray, *, thr: float | None = 0.5, step: float | None = None,
           min_variance: float | None = None, full: bool = False, double: bool = False,
           consistency: tuple[int, bool] | None = None, corrmap: bool = False):
@@ -203,3 +230,64 @@ def match(s0: np.ndarray, s1: np.ndarray, *, thr: float | None = 0.5, step: floa
                               df.ctypes.data, C.byref(is_float),
                               cm.ctypes.data if cm is not None else None))
     return (df if is_float.value else d16), cm
+
+
+def _cells(center: np.ndarray, halfwidth: np.ndarray, rows: int, cols: int):
+    center = np.ascontiguousarray(center, np.float32)
+    halfwidth = np.ascontiguousarray(halfwidth, np.int16)
+    if center.shape != (rows // 4, cols // 4) or halfwidth.shape != center.shape:
+        raise ValueError("center/halfwidth must be (rows // 4, cols // 4) cell arrays")
+    return center, halfwidth
+
+
+def search_window(d0: np.ndarray, d1: np.ndarray, flags: int, max_lr_diff: int,
+                  center: np.ndarray, halfwidth: np.ndarray) -> np.ndarray:
+    """Row search limited to each 4 x 4 cell's window [col - rint(c) - hw,
+    col - rint(c) + hw] (bicos.hpp:50-113 restricted; the Consistency back
+    search over [col - hw, col + hw])."""
+    d0 = np.ascontiguousarray(d0, np.uint32)
+    d1 = np.ascontiguousarray(d1, np.uint32)
+    rows, cols, words = d0.shape
+    center, halfwidth = _cells(center, halfwidth, rows, cols)
+    out = np.empty((rows, cols), np.int16)
+    _check(lib().oracle_search_window(d0.ctypes.data, d1.ctypes.data, rows, cols, words, flags,
+                                      max_lr_diff, center.ctypes.data, halfwidth.ctypes.data,
+                                      out.ctypes.data))
+    return out
+
+
+def match_windowed(s0: np.ndarray, s1: np.ndarray, center: np.ndarray, halfwidth: np.ndarray, *,
+                   hw_lo: int = 6, fallback: int = 1, thr: float | None = 0.5,
+                   step: float | None = None, min_variance: float | None = None,
+                   full: bool = False, double: bool = False,
+                   consistency: tuple[int, bool] | None = None, corrmap: bool = False):
+    """The learned-window BICOS::match on the CPU for given cell windows:
+    windowed search, NCC stage, then (fallback 1 = all, 2 = unsure) the
+    full-row search and NCC stage of the fallback set. Returns (disparity,
+    corrmap | None, fallback pixel count)."""
+    s0, p0, rstep, depth = _frames(s0)
+    s1, p1, _, _ = _frames(s1)
+    n, rows, cols = s0.shape
+    center, halfwidth = _cells(center, halfwidth, rows, cols)
+    cfg = OracleConfig(
+        int(thr is not None), float(thr or 0.0),
+        int(step is not None), float(step or 0.0),
+        int(min_variance is not None), float(min_variance or 0.0),
+        int(full), int(double),
+        int(consistency is not None),
+        int(consistency[0]) if consistency else 1,
+        int(consistency[1]) if consistency else 0,
+    )
+    d16 = np.empty((rows, cols), np.int16)
+    df = np.empty((rows, cols), np.float32)
+    cm = None
+    if corrmap and thr is not None:
+        cm = np.empty((rows, cols), np.float64 if double else np.float32)
+    is_float = C.c_int(0)
+    nfb = C.c_int64(0)
+    _check(lib().oracle_match_windowed(p0, p1, rstep, depth, n, rows, cols, C.byref(cfg),
+                                       center.ctypes.data, halfwidth.ctypes.data, int(hw_lo),
+                                       int(fallback), d16.ctypes.data, df.ctypes.data,
+                                       C.byref(is_float), cm.ctypes.data if cm is not None else None,
+                                       C.byref(nfb)))
+    return (df if is_float.value else d16), cm, int(nfb.value)

@@ -175,3 +175,15 @@ def test_window_weights_roundtrip(tmp_path):
     Wn.save(f, p, {"scenes": 12, "note": "x"})
     q, meta = Wn.load(f)
     assert np.array_equal(p, q) and meta == {"scenes": "12", "note": "'x'"}
+
+
+def test_search_chunk_cols():
+    """The full-row search's LDS chunk sizes (host logic, no device needed):
+    a 1920-column row of 128-bit descriptors is two fp4 chunks."""
+    import libbicos_amd as B
+    assert 2 * B.search_chunk_cols(B.SEARCH_MFMA, 4) == 1920
+    assert B.search_chunk_cols(B.SEARCH_VALU, 4) == 0
+    for w in (1, 2, 4, 8):
+        assert B.search_chunk_cols(B.SEARCH_MFMA_I8, w) > 0 and B.search_chunk_cols(B.SEARCH_MFMA, w) > 0
+    with pytest.raises(ValueError):
+        B.search_chunk_cols(B.SEARCH_MFMA, 3)

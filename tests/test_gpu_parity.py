@@ -57,17 +57,36 @@ LIMITED_N = [2, 3, 4, 5, 8, 9, 10, 16, 17, 18, 25, 32, 33, 34, 48, 64, 65]
 FULL_N = list(range(2, 17))
 
 
+def transform_kernels(gpu, fn):
+    """Runs fn with the library's per-kernel events on; the kernel names run."""
+    gpu.timing_enable(True)
+    try:
+        out = fn()
+        st = gpu.timing_read()
+    finally:
+        gpu.timing_enable(False)
+    return out, set(st)
+
+
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
 @pytest.mark.parametrize("full", [False, True])
-def test_transform_bit_exact(gpu, oracle, dtype, full):
-    rng = np.random.default_rng(100 + full + 2 * (dtype == np.uint16))
+@pytest.mark.parametrize("aligned", [True, False])
+def test_transform_bit_exact(gpu, oracle, dtype, full, aligned):
+    """Every n of either mode, at widths that are a multiple of 4 samples
+    (the 4-pixel-per-lane kernels, the production path; 2 pixels per lane for
+    u16 stacks deeper than 33 frames) and at odd widths (one pixel per lane);
+    the kernel that ran is checked by name."""
+    rng = np.random.default_rng(100 + full + 2 * (dtype == np.uint16) + 4 * aligned)
     for n in (FULL_N if full else LIMITED_N):
-        rows, cols = int(rng.integers(8, 40)), int(rng.integers(256, 1028))
+        rows = int(rng.integers(8, 40))
+        cols = 4 * int(rng.integers(64, 257)) if aligned else 2 * int(rng.integers(128, 514)) + 1
         for levels in (None, 3):          # 3 levels: equal samples and equal sums are common
             s = rand_stack(rng, n, rows, cols, dtype, levels)
             exp = oracle.transform(s, full=full)
-            got = host(gpu.transform(dev(s), int(full)), np.uint32)
-            assert np.array_equal(got, exp), (n, levels)
+            q = dev(s)
+            got, ran = transform_kernels(gpu, lambda: host(gpu.transform(q, int(full)), np.uint32))
+            assert ran == ({"transform"} if aligned else {"transform_scalar"}), (n, cols, ran)
+            assert np.array_equal(got, exp), (n, levels, cols)
 
 
 def test_transform_deep_u16_aligned(gpu, oracle):
@@ -156,15 +175,24 @@ def test_search_chunked_rows(gpu, oracle, search_impl):
 def test_search_fp4_random_sweep(gpu):
     """The fp4 matrix-core search (default for 64- to 256-bit descriptors)
     against the vector-ALU and int8 kernels, which the tests above pin to the
-    oracle: 150 seeded cases over row widths around the tile (32) and chunk
-    boundaries (1344 / 800 / 448 columns) up to 4096, every flag set and L/R
-    limit, and random, duplicate-heavy and sparse descriptors."""
+    oracle: 150 seeded cases over row widths around the tile (32) and around
+    one and two LDS chunks of either matrix-core kernel (the chunk sizes the
+    library reports for each descriptor width) up to 4096, every flag set and
+    L/R limit, and random, duplicate-heavy and sparse descriptors."""
     rng = np.random.default_rng(777)
-    edges = [1, 2, 31, 32, 33, 63, 447, 448, 449, 799, 800, 801, 1343, 1344, 1345, 1600, 1601, 4095, 4096]
+    edges = {}
+    for words in (2, 4, 8):
+        e = {1, 2, 31, 32, 33, 63, 4095, 4096}
+        for impl in (gpu.SEARCH_MFMA, gpu.SEARCH_MFMA_I8):
+            ch = gpu.search_chunk_cols(impl, words)
+            assert ch > 0
+            e |= {ch - 1, ch, ch + 1, 2 * ch - 1, 2 * ch, 2 * ch + 1}
+        edges[words] = sorted(w for w in e if 1 <= w <= 4096)
+    assert 1920 in edges[4] and 3328 in edges[2]   # the 1920-column row is two fp4 chunks
     try:
         for case in range(150):
             words = int(rng.choice([2, 4, 8]))
-            W = int(rng.choice(edges)) if rng.random() < 0.6 else int(rng.integers(1, 4097))
+            W = int(rng.choice(edges[words])) if rng.random() < 0.6 else int(rng.integers(1, 4097))
             H = int(rng.integers(1, 4))
             flags, lr = int(rng.integers(0, 4)), int(rng.integers(0, 5))
             kind = rng.integers(0, 3)
@@ -498,6 +526,30 @@ def test_match_full_size_1920x1200x33(gpu, oracle):
     print(f"valid {v.mean():.4f}, within 1px of ground truth {(np.abs(got[v] - gt[v]) <= 1).mean():.4f}")
 
 
+@pytest.mark.timeout(900)
+def test_match_full_size_4096x3000x64_consistency(gpu, oracle):
+    """The whole 4096x3000x64 u16 stress workload (256-bit descriptors,
+    Consistency(max_lr_diff 1), step 0.05, threshold 0.96, min_variance 2),
+    bit-exact against the oracle on every host core (about a minute)."""
+    import torch
+    from libbicos_amd import scene, window
+    spec = scene.SCENE_4096x3000_U16
+    dL, dR, _ = window.scene_device(spec, scene.BENCH_SEED)
+    cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.05, min_variance=2.0,
+                     variant=gpu.Variant.Consistency(1, False))
+    d, _ = gpu.match_device(dL, dR, cfg)
+    got = host(d)
+    L, R = host(dL), host(dR)
+    del dL, dR, d
+    torch.cuda.empty_cache()
+    print("oracle on the full 4096x3000x64 pair ...", flush=True)
+    exp, _ = oracle.match(L, R, thr=0.96, step=0.05, min_variance=2.0, consistency=(1, False))
+    assert same(got, exp)
+    v = ~np.isnan(got)
+    print(f"valid {v.mean():.4f}")
+    assert v.mean() > 0.5
+
+
 def test_workspace_reuse_across_sizes(gpu, oracle):
     """consecutive calls with shrinking and growing sizes reuse the workspace"""
     from libbicos_amd import scene
@@ -523,23 +575,25 @@ def test_numpy_match_api(gpu, oracle):
 
 def test_cpp_api_cli(gpu, oracle, tmp_path):
     """BICOS::match through the C++ API (bicos_cli) with the reference CLI's
-    option mapping: FULL unless --limited, min variance default 1.0."""
+    option mapping: FULL unless --limited; min_variance only when -v is given
+    (src/cli.cpp:154-156)."""
     from libbicos_amd import scene
     L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=107, height=20, width=640, frames=24)
     (tmp_path / "l.raw").write_bytes(L.tobytes())
     (tmp_path / "r.raw").write_bytes(R.tobytes())
     cli = os.path.join(ROOT, "libbicos_amd", "bicos_cli")
     out = tmp_path / "d.raw"
-    cmd = [cli, "--left", str(tmp_path / "l.raw"), "--right", str(tmp_path / "r.raw"),
-           "-n", "24", "--rows", "20", "--cols", "640", "--depth", "16", "--limited",
-           "-t", "0.9", "-s", "0.1", "-m", "2", "--corrmap", str(tmp_path / "c.raw"), "-o", str(out)]
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    got = np.frombuffer(out.read_bytes(), np.float32).reshape(20, 640)
-    got_cm = np.frombuffer((tmp_path / "c.raw").read_bytes(), np.float32).reshape(20, 640)
-    exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.1, min_variance=1.0, full=False,
-                               consistency=(2, False), corrmap=True)
-    assert same(got, exp) and same(got_cm, exp_cm)
+    base = [cli, "--left", str(tmp_path / "l.raw"), "--right", str(tmp_path / "r.raw"),
+            "-n", "24", "--rows", "20", "--cols", "640", "--depth", "16", "--limited",
+            "-t", "0.9", "-s", "0.1", "-m", "2", "--corrmap", str(tmp_path / "c.raw"), "-o", str(out)]
+    for extra, minvar in (([], None), (["-v", "1.5"], 1.5)):
+        r = subprocess.run(base + extra, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = np.frombuffer(out.read_bytes(), np.float32).reshape(20, 640)
+        got_cm = np.frombuffer((tmp_path / "c.raw").read_bytes(), np.float32).reshape(20, 640)
+        exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.1, min_variance=minvar, full=False,
+                                   consistency=(2, False), corrmap=True)
+        assert same(got, exp) and same(got_cm, exp_cm), extra
 
 
 def test_python_cli_png_stacks(gpu, oracle, tmp_path):
@@ -565,7 +619,8 @@ def test_python_cli_png_stacks(gpu, oracle, tmp_path):
            "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=ROOT)
     assert r.returncode == 0, r.stderr
-    exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.1, min_variance=1.0, full=False, corrmap=True)
+    # no -v: no min_variance (src/cli.cpp:154-156)
+    exp, exp_cm = oracle.match(L, R, thr=0.9, step=0.1, min_variance=None, full=False, corrmap=True)
     got = imageio.read_tiff(str(tmp_path / "o" / "disp.tiff"))
     got_cm = imageio.read_tiff(str(tmp_path / "o" / "disp-corrmap.tiff"))
     assert same(got, exp) and same(got_cm, exp_cm)
@@ -574,6 +629,43 @@ def test_python_cli_png_stacks(gpu, oracle, tmp_path):
     lines = (tmp_path / "o" / "disp.xyz").read_text().splitlines()
     assert len(lines) == len(pts_exp)
     assert lines[:50] == [f"{x:g} {y:g} {z:g}" for x, y, z in pts_exp[:50].tolist()]
+
+
+def test_async_streams_and_workspace_growth(gpu, oracle):
+    """match_device on two non-default streams in turn, the workspace growing
+    between calls (larger images on later calls): each call is ordered after
+    the previous user of the shared workspace and equals the oracle, and a
+    call that does not grow the workspace returns while work queued ahead of
+    it on its stream is still running (no host synchronisation; the
+    reference creates, synchronises and destroys streams per call,
+    src/impl/cuda.cu.in:83-92)."""
+    import torch
+    from libbicos_amd import scene
+    cfg = gpu.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    sizes = [(16, 256), (24, 640), (16, 256), (40, 1024), (40, 1024), (8, 128)]
+    grows = [False, True, False, True, False, False]   # call 0 follows a released workspace
+    pairs = [scene.generate(scene.SCENE_1920x1200_U16, seed=300 + k, height=h, width=w, frames=33)[:2]
+             for k, (h, w) in enumerate(sizes)]
+    exp = [oracle.match(L, R, thr=0.96, step=0.1, min_variance=2.0)[0] for L, R in pairs]
+    devs = [(dev(L), dev(R)) for L, R in pairs]
+    busy = torch.rand(4096, 4096, device="cuda")
+    torch.cuda.synchronize()
+    gpu.release_workspace()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs, early = [], []
+    for k, (dl, dr) in enumerate(devs):
+        s = streams[k % 2]
+        with torch.cuda.stream(s):
+            x = busy
+            for _ in range(12):          # tens of ms of matrix products ahead of the call
+                x = torch.mm(x, busy)
+            d, _ = gpu.match_device(dl, dr, cfg, stream=s)
+            early.append(not s.query())
+        outs.append((d, x))
+    torch.cuda.synchronize()
+    for k, ((d, _), e) in enumerate(zip(outs, exp)):
+        assert same(host(d), e), k
+    assert all(e for e, g in zip(early, grows) if not g), early
 
 
 def test_errors_on_device(gpu):

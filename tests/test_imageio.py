@@ -133,8 +133,14 @@ def test_q_matrix_yaml_and_xml(tmp_path):
 
 def test_cli_option_mapping():
     import libbicos_amd as B
+    # min_variance only when -v is given (src/cli.cpp:154-156: cxxopts count()
+    # does not count the option's default value)
     c = cli.config_from_args(cli.parse_args(["f"]), B)
-    assert (c.nxcorr_threshold, c.min_variance, c.subpixel_step, c.mode) == (0.75, 1.0, None, B.TransformMode.Full)
+    assert (c.nxcorr_threshold, c.min_variance, c.subpixel_step, c.mode) == (0.75, None, None, B.TransformMode.Full)
+    c = cli.config_from_args(cli.parse_args(["f", "-v", "1.0"]), B)
+    assert c.min_variance == 1.0
+    c = cli.config_from_args(cli.parse_args(["f", "--variance", "2.5"]), B)
+    assert c.min_variance == 2.5
     assert isinstance(c.variant, B.Variant.NoDuplicates) and c.precision == B.Precision.Single
     c = cli.config_from_args(cli.parse_args(["f", "-t", "0", "-v", "0"]), B)
     assert c.nxcorr_threshold is None and c.min_variance is None

@@ -172,6 +172,147 @@ def test_search_planted(oracle):
     assert got1[0, 0] == 0 and got1[0, 1] == 0
 
 
+# ------------------------------------------------------ windowed search
+# The learned-window path's search: bicos.hpp:50-113 with the scan limited
+# to [k - hw, k + hw], k = col - rint(centre of the pixel's 4 x 4 cell),
+# clipped to the row; the Consistency back search over [col - hw, col + hw].
+
+def brute_search_window(d0, d1, flags, max_lr, center, hw):
+    """Per-pixel numpy re-formulation: the window's costs, first minimum via
+    argmin, duplicates via a count of the minimum."""
+    rows, cols, _ = d0.shape
+    Hq, Wq = rows // 4, cols // 4
+    out = np.full((rows, cols), -32768, np.int16)
+
+    def best_in(q, row, lo, hi):
+        if lo > hi:
+            return None
+        cost = np.bitwise_count(row[lo:hi + 1] ^ q[None, :]).sum(-1).astype(np.int64)
+        m = cost.min()
+        if flags & 1 and (cost == m).sum() > 1:
+            return None
+        return lo + int(cost.argmin())
+
+    for r in range(rows):
+        for c in range(cols):
+            cell = (min(r // 4, Hq - 1), min(c // 4, Wq - 1))
+            k = c - int(np.rint(np.float32(center[cell])))
+            h = int(hw[cell])
+            b = best_in(d0[r, c], d1[r], max(k - h, 0), min(k + h, cols - 1))
+            if b is None:
+                continue
+            if flags & 2:
+                rc = best_in(d1[r, b], d0[r], max(c - h, 0), min(c + h, cols - 1))
+                if rc is None or abs(c - rc) > max_lr:
+                    continue
+                out[r, c] = (c + rc) // 2 - b
+            else:
+                out[r, c] = c - b
+    return out
+
+
+def random_windows(rng, rows, cols, widths=(0, 6, 12)):
+    """Cell windows that cover the interior, both row ends (clipped), fully
+    outside the row (empty) and exact half-way centres (rint: half to even)."""
+    Hq, Wq = rows // 4, cols // 4
+    center = rng.uniform(-20, 40, size=(Hq, Wq)).astype(np.float32)
+    kind = rng.integers(0, 5, size=(Hq, Wq))
+    center[kind == 1] = -cols - 30.0                          # window past the right end
+    center[kind == 2] = cols + 30.0                           # window before column 0
+    center[kind == 3] = np.round(center[kind == 3]) + 0.5     # ties
+    hw = rng.choice(np.array(widths, np.int16), size=(Hq, Wq)).astype(np.int16)
+    return center, hw
+
+
+@pytest.mark.parametrize("words", [1, 2, 4, 8])
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_search_window_matches_bruteforce(oracle, words, flags):
+    rng = np.random.default_rng(700 + 10 * words + flags)
+    rows, cols = 9, 70       # a partial last cell row and column (9 = 2*4 + 1, 70 = 17*4 + 2)
+    pool = rng.integers(0, 2**32, size=(12, words), dtype=np.uint64).astype(np.uint32)
+    d0 = pool[rng.integers(0, 12, size=(rows, cols))]
+    d1 = pool[rng.integers(0, 12, size=(rows, cols))]
+    center, hw = random_windows(rng, rows, cols)
+    for lr in (0, 2):
+        exp = brute_search_window(d0, d1, flags, lr, center, hw)
+        got = oracle.search_window(d0, d1, flags, lr, center, hw)
+        assert np.array_equal(got, exp)
+    # the cases the windows were built for occur
+    assert (exp == -32768).any() and (exp != -32768).any()
+
+
+def test_search_window_rint_half_even(oracle):
+    """Centres 2.5 and 3.5 round to 2 and 4 (half to even): a single exact
+    match at column col - 2 (resp. col - 4) is found with hw = 0."""
+    rows, cols = 4, 16
+    rng = np.random.default_rng(3)
+    d1 = rng.integers(0, 2**32, size=(rows, cols, 2), dtype=np.uint64).astype(np.uint32)
+    for c, shift in ((2.5, 2), (3.5, 4), (-2.5, -2), (2.4999, 2), (2.5001, 3)):
+        d0 = np.roll(d1, shift, axis=1)
+        center = np.full((1, 4), c, np.float32)
+        hw = np.zeros((1, 4), np.int16)
+        got = oracle.search_window(d0, d1, 1, 0, center, hw)
+        cols_ok = np.arange(cols)[(np.arange(cols) - shift >= 0) & (np.arange(cols) - shift < cols)]
+        assert (got[:, cols_ok] == shift).all(), (c, shift)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_search_window_full_row_is_plain_search(oracle, flags):
+    """A window that spans the row at every pixel (centre 0, hw >= cols) is the
+    plain search; so is one centred on each pixel's own column range."""
+    rng = np.random.default_rng(40 + flags)
+    rows, cols = 8, 96
+    pool = rng.integers(0, 2**32, size=(30, 4), dtype=np.uint64).astype(np.uint32)
+    d0 = pool[rng.integers(0, 30, size=(rows, cols))]
+    d1 = pool[rng.integers(0, 30, size=(rows, cols))]
+    center = np.zeros((rows // 4, cols // 4), np.float32)
+    hw = np.full(center.shape, 2 * cols, np.int16)
+    for lr in (0, 3):
+        assert np.array_equal(oracle.search_window(d0, d1, flags, lr, center, hw),
+                              oracle.search(d0, d1, flags, lr))
+
+
+def test_match_windowed_fallback_semantics(oracle):
+    """The learned-window match on a scene: with whole-row windows it is the
+    plain match for every fallback mode; with useless windows (far off the
+    true disparity) fallback ALL gives the plain output at every pixel, NONE
+    gives the windowed search followed by the NCC stage, and the fallback set
+    is the NCC-rejected pixels minus those whose left variance rejects every
+    match."""
+    from libbicos_amd import scene
+    L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=103, height=24, width=256, frames=12)
+    kw = dict(thr=0.9, step=0.25, min_variance=2.0)
+    plain, plain_cm = oracle.match(L, R, corrmap=True, **kw)
+    Hq, Wq = 6, 64
+    whole = (np.zeros((Hq, Wq), np.float32), np.full((Hq, Wq), 600, np.int16))
+    for fb in (0, 1, 2):
+        got, cm, nfb = oracle.match_windowed(L, R, *whole, fallback=fb, corrmap=True, **kw)
+        assert np.array_equal(got, plain, equal_nan=True) and np.array_equal(cm, plain_cm, equal_nan=True)
+    off = (np.full((Hq, Wq), -150.0, np.float32), np.full((Hq, Wq), 6, np.int16))
+    got_all, _, nfb = oracle.match_windowed(L, R, *off, fallback=1, **kw)
+    assert np.array_equal(got_all, plain, equal_nan=True)
+    got_none, _, nfb0 = oracle.match_windowed(L, R, *off, fallback=0, **kw)
+    words = oracle.descriptor_words(12, False)
+    ws = oracle.search_window(oracle.transform(L, False, words), oracle.transform(R, False, words),
+                              1, -1, *off)
+    exp_none, _ = oracle.agree_subpixel(ws, L, R, 0.9, 0.25, minvar=2.0 * 12)
+    assert np.array_equal(got_none, exp_none, equal_nan=True) and nfb0 == 0
+    # fallback set = invalid after the windowed pass, minus the hopeless
+    # pixels (float64 left variance; no pixel of this scene is within FP32
+    # rounding of the limit)
+    v0 = L.astype(np.float64).var(axis=0) * 12
+    hopeless = v0 < 2.0 * 12
+    assert nfb == int((np.isnan(exp_none) & ~hopeless).sum()) > 0
+    # UNSURE: only cells with the wide window are searched again
+    hw_mixed = np.where(np.arange(Wq)[None, :] % 2 == 0, 6, 12).astype(np.int16) * np.ones((Hq, 1), np.int16)
+    got_u, _, nfb_u = oracle.match_windowed(L, R, off[0], hw_mixed, hw_lo=6, fallback=2, **kw)
+    got_n, _, _ = oracle.match_windowed(L, R, off[0], hw_mixed, hw_lo=6, fallback=0, **kw)
+    wide = np.repeat(np.repeat(hw_mixed == 12, 4, 0), 4, 1)
+    assert np.array_equal(got_u[~wide], got_n[~wide], equal_nan=True)
+    redo = wide & np.isnan(got_n)
+    assert np.array_equal(got_u[redo], plain[redo], equal_nan=True)
+
+
 # ---------------------------------------------------------------- NCC
 
 def ncc64(a, b, minvar=None):
@@ -230,6 +371,29 @@ def _linear_right_stack(rng, n, cols):
     beta = 2 * rng.integers(1, 21, n)                # even slopes
     c = np.arange(cols)
     return alpha, beta, (alpha[:, None] + beta[:, None] * c[None, :]).astype(np.uint16)
+
+
+def test_subpixel_cast_wraps(oracle):
+    """The float -> u8/u16 cast of an interpolated sample (agree.hpp:164) is
+    undefined in C++ when the parabola leaves the sample range; the oracle
+    takes x86-64 GCC's conversion to int32 after rint, keeping the low bits.
+    Known answers, hand-computed at x = 0.6 (a saturating cast would give
+    65535 / 0 / 255 / 0, a truncating one 4320 / ...), and rint's half to
+    even inside the range."""
+    # (0, 60000, 65535): -27232.5 * 0.36 + 32767.5 * 0.6 + 60000 = 69856.8 -> 69857 -> 69857 - 65536
+    assert oracle.interp_sample(0, 60000, 65535, 0.6, np.uint16) == 4321
+    # (65535, 5000, 0): 27767.5 * 0.36 - 19660.5 + 5000 = -4664.2 -> -4664 -> 65536 - 4664
+    assert oracle.interp_sample(65535, 5000, 0, 0.6, np.uint16) == 60872
+    # (0, 230, 255): -102.5 * 0.36 + 76.5 + 230 = 269.6 -> 270 -> 270 - 256
+    assert oracle.interp_sample(0, 230, 255, 0.6, np.uint8) == 14
+    # (255, 20, 0): 107.5 * 0.36 - 76.5 + 20 = -17.8 -> -18 -> 256 - 18
+    assert oracle.interp_sample(255, 20, 0, 0.6, np.uint8) == 238
+    # a line through (1, 2, 3) at x = 0.5 is 2.5 -> 2; (3, 4, 5) at +-0.5 is 4.5 -> 4 and 3.5 -> 4
+    assert oracle.interp_sample(1, 2, 3, 0.5) == 2
+    assert oracle.interp_sample(3, 4, 5, 0.5) == 4
+    assert oracle.interp_sample(3, 4, 5, -0.5) == 4
+    # the grid's own end points reproduce the columns
+    assert oracle.interp_sample(17, 400, 9000, -1.0) == 17 and oracle.interp_sample(17, 400, 9000, 0.0) == 400
 
 
 def test_subpixel_half_pixel_shift(oracle):

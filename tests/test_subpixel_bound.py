@@ -1,0 +1,165 @@
+"""The error bound behind the bit-exact pruned subpixel stage, checked on the
+CPU against the oracle.
+
+The pruned kernel (libbicos_amd/csrc/agree_moments.hpp,
+k_agree_subpixel_pruned) never evaluates most grid points with the
+reference's arithmetic (include/impl/cpu/agree.hpp:143-171). It evaluates the
+closed form r(x) of the correlation of the left samples against the
+*unrounded* parabola samples, from exact moments, and trusts
+
+    |r'(x) - r(x)| <= (sqrt(1 - r^2) d + d^2 / (s - d)) / (s - d) + 1e-6 + 2.5e-7 n
+
+where r'(x) is the reference's FP32 correlation of the left samples against
+the rounded, cast samples, d = 0.52 sqrt(n) bounds the rounding vector and
+s = sqrt(var1(x)) its unrounded right variance. The kernel relies on it only
+where s > 2 d and every frame's parabola stays inside [2, max - 2] (integer
+reach y1 -/+ ceil(|y2 - y0| / 4)); elsewhere it scores every point exactly.
+
+This test restates that inequality (not the kernel's code) and checks it at
+every grid point of more than a million pixels drawn to stress it:
+correlations near 1 (where sqrt(1 - r^2) vanishes), near the 0.96
+threshold, low-contrast pixels near the min_variance scale (s close to 2 d),
+saturated pixels at the edge of the range, random pixels, u8 and u16, at the
+step and depth of each benchmark configuration.
+"""
+import numpy as np
+import pytest
+
+
+def exact_closed_form(u, y, grid):
+    """r(x) and var1(x) of the unrounded parabola samples, in float64 from
+    exact integer moments: u (P, n), y (P, 3, n) -> (r, var1), each (P, K)."""
+    n = u.shape[1]
+    ui = u.astype(np.int64)
+    yi = y.astype(np.int64)
+    su = ui.sum(1).astype(np.float64)
+    suu = (ui * ui).sum(1).astype(np.float64)
+    sy = yi.sum(2).astype(np.float64)                                   # (P, 3)
+    suy = (ui[:, None, :] * yi).sum(2).astype(np.float64)               # (P, 3)
+    syy = np.einsum("pjt,pkt->pjk", yi, yi).astype(np.float64)          # (P, 3, 3), exact in int64
+    var0 = suu - su * su / n
+    uY = suy - su[:, None] * sy / n
+    G = syy - sy[:, :, None] * sy[:, None, :] / n
+    # parabola through (-1, y0), (0, y1), (1, y2): p(x) = a x^2 + b x + c
+    # with a = (y0 - 2 y1 + y2) / 2, b = (y2 - y0) / 2, c = y1: weights per column
+    x = grid.astype(np.float64)[None, :]
+    w0 = 0.5 * (x * x - x)
+    w1 = 1.0 - x * x
+    w2 = 0.5 * (x * x + x)
+    W = np.stack([w0 + 0 * x, w1 + 0 * x, w2 + 0 * x], axis=-1)[0]      # (K, 3)
+    cov = uY @ W.T                                                      # (P, K)
+    var1 = np.einsum("kj,pjl,kl->pk", W, G, W)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = cov / np.sqrt(var0[:, None] * np.maximum(var1, 1e-300))
+    return r, var1, var0
+
+
+def reach_in_range(y, vmax):
+    """The kernel's integer reach test: every frame's parabola on [-1, 1]
+    inside [2, vmax - 2]."""
+    y0, y1, y2 = (y[:, j, :].astype(np.int64) for j in range(3))
+    r4 = (np.abs(y2 - y0) + 3) >> 2
+    lo = np.minimum(np.minimum(y0, y2), y1 - r4).min(1)
+    hi = np.maximum(np.maximum(y0, y2), y1 + r4).max(1)
+    return (lo >= 2) & (hi <= vmax - 2)
+
+
+def draw_pixels(rng, P, n, vmax, kind):
+    """(left (P, n), right columns (P, 3, n)) of one family."""
+    if kind == "random":
+        return (rng.integers(0, vmax + 1, (P, n)), rng.integers(0, vmax + 1, (P, 3, n)))
+    # a speckle profile over 5 columns per frame; the left pixel sees it at a
+    # sub-pixel offset f (linear interpolation), the right columns at -1, 0, 1
+    if kind == "low_contrast":
+        mid = rng.uniform(0.2, 0.8, (P, 1, 1)) * vmax
+        amp = rng.uniform(0.5, 6.0, (P, 1, 1)) * (vmax / 65535.0 if vmax > 255 else 0.05)
+        noise = rng.uniform(0.0, 1.5, (P, 1, 1)) * (1 if vmax > 255 else 0.2)
+    elif kind == "saturated":
+        mid = np.where(rng.random((P, 1, 1)) < 0.5, 0.03, 0.97) * vmax
+        amp = rng.uniform(0.005, 0.05, (P, 1, 1)) * vmax
+        noise = rng.uniform(0.0, 0.01, (P, 1, 1)) * vmax
+    else:
+        mid = rng.uniform(0.3, 0.7, (P, 1, 1)) * vmax
+        amp = rng.uniform(0.02, 0.3, (P, 1, 1)) * vmax
+        # near 1: little noise; near the 0.96 threshold: more
+        scale = 0.002 if kind == "correlated" else rng.uniform(0.05, 0.35, (P, 1, 1))
+        noise = scale * amp
+    prof = mid + amp * rng.standard_normal((P, n, 5))                  # columns -2..2
+    f = rng.uniform(-0.5, 0.5, (P, 1))
+    fl = np.floor(f).astype(np.int64)                                   # -1 or 0
+    g = f - fl
+    idx = 2 + fl
+    left = (np.take_along_axis(prof, np.repeat(idx[:, :, None], 1, 2).repeat(n, 1), 2)[..., 0] * (1 - g)
+            + np.take_along_axis(prof, np.repeat(idx[:, :, None] + 1, 1, 2).repeat(n, 1), 2)[..., 0] * g)
+    right = np.stack([prof[:, :, 1], prof[:, :, 2], prof[:, :, 3]], axis=1)
+    left = left + noise[..., 0] * rng.standard_normal((P, n))
+    right = right + noise * rng.standard_normal((P, 3, n))
+    return (np.clip(np.rint(left), 0, vmax), np.clip(np.rint(right), 0, vmax))
+
+
+CASES = [  # (dtype, n, step, pixels per family): the c1, c4 and c0 configurations
+    (np.uint16, 33, 0.1, 130_000),
+    (np.uint16, 64, 0.05, 30_000),
+    (np.uint8, 12, 0.25, 40_000),
+]
+FAMILIES = ["correlated", "near_threshold", "low_contrast", "saturated", "random"]
+
+
+@pytest.mark.parametrize("dtype,n,step,P", CASES)
+def test_pruned_subpixel_bound_holds(oracle, dtype, n, step, P):
+    rng = np.random.default_rng(9000 + n)
+    vmax = int(np.iinfo(dtype).max)
+    grid = oracle.subpixel_grid(step)
+    d = 0.52 * np.sqrt(n)
+    slack = 1e-6 + 2.5e-7 * n
+    checked, worst, near = 0, 0.0, 0
+    for kind in FAMILIES:
+        u, y = draw_pixels(rng, P, n, vmax, kind)
+        u, y = u.astype(dtype), y.astype(dtype)
+        r, var1, var0 = exact_closed_form(u, y, grid)
+        ref = oracle.subpixel_curve(u, y, step).astype(np.float64)    # the reference's FP32 values
+        s = np.sqrt(np.maximum(var1, 0.0))
+        use = (s > 2 * d) & (var0[:, None] > 0) & reach_in_range(y, vmax)[:, None]
+        sd = np.where(use, s - d, 1.0)
+        bound = (np.sqrt(np.maximum(1.0 - r * r, 0.0)) * d + d * d / sd) / sd + slack
+        err = np.abs(ref - r)
+        bad = use & ~(err <= bound)
+        assert not bad.any(), (kind, int(bad.sum()), float((err - bound)[bad].max()))
+        checked += int(use.sum())
+        if use.any():
+            worst = max(worst, float((err / bound)[use].max()))
+            near += int((use & (np.abs(r - 0.96) < 0.01)).sum())
+    print(f"{dtype.__name__} n={n}: {checked} (pixel, point) pairs, max err/bound {worst:.3f}, "
+          f"{near} within 0.01 of the 0.96 threshold")
+    # the families reach the regions the bound is for
+    assert checked >= 0.2 * len(FAMILIES) * P * len(grid)
+    assert near > 1000 and worst > 0.05
+
+
+def test_pixel_count_over_a_million():
+    assert sum(P * len(FAMILIES) for _, _, _, P in CASES) >= 1_000_000
+
+
+def test_bound_is_not_vacuous(oracle):
+    """The checks above can fail: without its sqrt(1 - r^2) d term the bound
+    is broken on every family, and half of the full bound is broken on
+    well-correlated pixels (the bound is within a factor of about two of the
+    worst error seen)."""
+    rng = np.random.default_rng(11)
+    n, step = 33, 0.1
+    grid = oracle.subpixel_grid(step)
+    d = 0.52 * np.sqrt(n)
+    halves = 0
+    for kind in ("low_contrast", "correlated", "near_threshold"):
+        u, y = draw_pixels(rng, 50_000, n, 65535, kind)
+        u, y = u.astype(np.uint16), y.astype(np.uint16)
+        r, var1, var0 = exact_closed_form(u, y, grid)
+        ref = oracle.subpixel_curve(u, y, step).astype(np.float64)
+        s = np.sqrt(np.maximum(var1, 0.0))
+        use = (s > 2 * d) & (var0[:, None] > 0) & reach_in_range(y, 65535)[:, None]
+        err, sd, rr = np.abs(ref - r)[use], (s - d)[use], r[use]
+        w = np.sqrt(np.maximum(1.0 - rr * rr, 0.0))
+        slack = 1e-6 + 2.5e-7 * n
+        assert (err > (d * d / sd) / sd + slack).any(), kind
+        halves += int((err > 0.5 * (w * d + d * d / sd) / sd).sum())
+    assert halves > 0

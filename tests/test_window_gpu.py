@@ -318,6 +318,131 @@ def _fallback_invariant(W, B, variant, subpix, hw, moments):
           f"agree {(a == b)[~inv_a].mean():.4f}")
 
 
+# ----------------------------------------- windowed path against the oracle
+
+def dev_np(a):
+    import torch
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint16:
+        return torch.from_numpy(a.view(np.int16)).cuda().view(torch.uint16)
+    if a.dtype == np.uint32:
+        return torch.from_numpy(a.view(np.int32)).cuda()
+    return torch.from_numpy(a).cuda()
+
+
+def oracle_windows(rng, rows, cols, widths=(0, 6, 12)):
+    """Cell windows covering the interior, both clipped row ends, empty
+    windows (wholly outside the row) and exact half-way centres."""
+    Hq, Wq = rows // 4, cols // 4
+    center = rng.uniform(-20, 300, size=(Hq, Wq)).astype(np.float32)
+    kind = rng.integers(0, 6, size=(Hq, Wq))
+    center[kind == 1] = -cols - 30.0
+    center[kind == 2] = cols + 30.0
+    center[kind == 3] = np.round(center[kind == 3]) + 0.5
+    center[kind == 4] = rng.uniform(-15, 15, size=int((kind == 4).sum()))   # clipped at col 0 / W-1
+    hw = rng.choice(np.array(widths, np.int16), size=(Hq, Wq)).astype(np.int16)
+    return center, hw
+
+
+@pytest.mark.parametrize("words", [1, 2, 4, 8])
+def test_search_windowed_matches_oracle(W, oracle, words):
+    """The windowed search kernel against the oracle's window-restricted
+    bicos search (bicos.hpp:50-113 restricted): half-widths 0 / 6 / 12,
+    windows clipped at both row ends, empty windows, half-way centres, every
+    flag set and L/R limit, duplicate-rich descriptors, ragged sizes."""
+    rng = np.random.default_rng(500 + words)
+    for rows, cols in ((9, 70), (12, 1920), (5, 333)):
+        pool = rng.integers(0, 2**32, size=(24, words), dtype=np.uint64).astype(np.uint32)
+        d0 = pool[rng.integers(0, 24, size=(rows, cols))]
+        d1 = pool[rng.integers(0, 24, size=(rows, cols))]
+        center, hw = oracle_windows(rng, rows, cols)
+        q0, q1, qc, qh = dev_np(d0), dev_np(d1), dev_np(center), dev_np(hw)
+        for flags in (0, 1, 2, 3):
+            for lr in (0, 2):
+                got = host(W.search_windowed(q0, q1, flags, lr, qc, qh))
+                exp = oracle.search_window(d0, d1, flags, lr, center, hw)
+                assert np.array_equal(got, exp), (rows, cols, flags, lr)
+
+
+def _windowed_vs_oracle(W, oracle, L, R, params, opts, cfg_kw, okw, corrmap):
+    """bicos_hip_match_windowed against the oracle's windowed match fed with
+    the windows the network predicts for the same stacks."""
+    import libbicos_amd as B
+    c, hw, _ = W.predict(L, R, params, opts)
+    center, halfw = host(c), host(hw)
+    cfg = B.Config(**cfg_kw)
+    d, cm, st = W.match_windowed_device(L, R, cfg, params, opts, want_corrmap=corrmap, stats=True)
+    exp, exp_cm, nfb = oracle.match_windowed(host(L), host(R), center, halfw, hw_lo=opts.hw_lo,
+                                             fallback=opts.fallback, corrmap=corrmap, **okw)
+    got = host(d)
+    assert got.dtype == exp.dtype and np.array_equal(got, exp, equal_nan=True)
+    if corrmap and exp_cm is not None:
+        assert np.array_equal(host(cm), exp_cm, equal_nan=True)
+    return got, st, nfb, halfw
+
+
+@pytest.mark.parametrize("fallback", ["all", "none", "unsure"])
+@pytest.mark.parametrize("stage", ["subpixel", "agree", "search_only", "double"])
+def test_windowed_match_matches_oracle_c1(W, oracle, fallback, stage):
+    """1920-wide c1 crops (33 frames u16, threshold 0.96, min_variance 2,
+    step 0.1): the learned path with the committed network's windows (half-
+    widths 6 / 12) bit for bit equal to the oracle's windowed search ->
+    NCC stage -> fallback full-row search -> NCC stage; the fallback set has
+    the oracle's size. Also without subpixel step, without threshold (the
+    windowed search alone) and in FP64."""
+    import libbicos_amd as B
+    params = W.params_device(W.load()[0]) if os.path.exists(W.DEFAULT_WEIGHTS) else W.params_device(W.init_params(9))
+    fb = {"all": W.FALLBACK_ALL, "none": W.FALLBACK_NONE, "unsure": W.FALLBACK_UNSURE}[fallback]
+    opts = W.WindowOptions(fallback=fb)
+    cfg_kw = dict(BENCH_CFG)
+    okw = dict(thr=0.96, step=0.1, min_variance=2.0)
+    if stage == "agree":
+        cfg_kw.pop("subpixel_step"); okw["step"] = None
+    elif stage == "search_only":
+        cfg_kw = dict(nxcorr_threshold=None); okw = dict(thr=None)
+    elif stage == "double":
+        cfg_kw["precision"] = B.Precision.Double; okw["double"] = True
+    L, R = c1_crop(W, 61, H=48, Wd=1920)
+    got, st, nfb, halfw = _windowed_vs_oracle(W, oracle, L, R, params, opts, cfg_kw, okw, corrmap=stage != "search_only")
+    assert (halfw == 6).any() and (halfw == 12).any() or not os.path.exists(W.DEFAULT_WEIGHTS)
+    if stage != "search_only" and fallback != "none":
+        assert st["fallback_pixels"] == nfb > 0
+    print(f"{stage} {fallback}: fallback {nfb / got.size:.4f}")
+
+
+def test_windowed_match_matches_oracle_untrained(W, oracle):
+    """An untrained network's windows (centres all over the range, clipped
+    and empty windows) with narrow half-widths: the learned path still
+    equals the oracle's windowed match bit for bit (fallback ALL: the plain
+    output wherever the windowed pass is rejected)."""
+    params = W.params_device(W.init_params(3))
+    L, R = c1_crop(W, 62, H=32, Wd=640)
+    opts = W.WindowOptions(hw_lo=2, hw_hi=5, fallback=W.FALLBACK_ALL)
+    _windowed_vs_oracle(W, oracle, L, R, params, opts, dict(BENCH_CFG),
+                        dict(thr=0.96, step=0.1, min_variance=2.0), corrmap=True)
+
+
+def test_windowed_match_matches_oracle_4096_band(W, oracle):
+    """The learned form of the 4096x3000x64 stress workload on a 16-row band:
+    64 frames (256-bit descriptors), Consistency(max_lr_diff 1), step 0.05,
+    threshold 0.96, min_variance 2, K = 144 bins, fallback ALL; bit for bit
+    equal to the oracle's windowed match; and every pixel the learned path
+    rejects is rejected by the oracle's plain match."""
+    import libbicos_amd as B
+    from libbicos_amd import scene
+    spec = scene.SCENE_4096x3000_U16
+    L, R, _ = W.scene_device(spec, 0x600DF00D, height=16)
+    params = W.params_device(W.load()[0]) if os.path.exists(W.DEFAULT_WEIGHTS) else W.params_device(W.init_params(9))
+    opts = W.WindowOptions(bins=144, fallback=W.FALLBACK_ALL)
+    cfg_kw = dict(nxcorr_threshold=0.96, subpixel_step=0.05, min_variance=2.0,
+                  variant=B.Variant.Consistency(max_lr_diff=1, no_dupes=False))
+    okw = dict(thr=0.96, step=0.05, min_variance=2.0, consistency=(1, False))
+    got, st, nfb, _ = _windowed_vs_oracle(W, oracle, L, R, params, opts, cfg_kw, okw, corrmap=False)
+    assert got.shape == (16, 4096) and st["fallback_pixels"] == nfb
+    plain, _ = oracle.match(host(L), host(R), **okw)
+    assert not (np.isnan(got) & np.isfinite(plain)).any()
+
+
 def test_windowed_errors(W):
     import torch
     import libbicos_amd as B
@@ -356,17 +481,21 @@ def test_trained_windows_contain_labels(W):
     assert hit >= 0.97 and hit0 <= 0.5
 
 
-def test_trained_agreement_full_size_1920x1200x33(W):
+@pytest.mark.parametrize("prec", ["moments", "half_mfma"])
+def test_trained_agreement_full_size_1920x1200x33(W, prec):
     """1920x1200x33 u16, unseen scenes: >= 99.5 % of the plain-valid pixels
-    are valid in the learned path and within 0.5 px."""
+    are valid in the learned path and within 0.5 px. MOMENTS with fallback
+    ALL is the configuration the benchmark reports."""
     import libbicos_amd as B
     from libbicos_amd import scene
     params, _ = trained(W)
-    cfg = B.Config(precision=B.Precision.HalfMFMA, **BENCH_CFG)
+    cfg = B.Config(precision=B.Precision.Moments if prec == "moments" else B.Precision.HalfMFMA,
+                   **BENCH_CFG)
     for k in range(2):
         L, R, _ = W.scene_device(scene.SCENE_1920x1200_U16, W.EVAL_SEED0 + 100 + k)
         a = host(B.match_device(L, R, B.Config(**BENCH_CFG))[0])
-        d, _, st = W.match_windowed_device(L, R, cfg, params, stats=True)
+        d, _, st = W.match_windowed_device(L, R, cfg, params, W.WindowOptions(fallback=W.FALLBACK_ALL),
+                                           stats=True)
         b = host(d)
         va = np.isfinite(a)
         ok = np.isfinite(b) & (np.abs(b - a) <= 0.5)
