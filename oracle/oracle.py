@@ -71,6 +71,8 @@ def lib():
         L.oracle_required_bits.argtypes = [i32, i32]
         L.oracle_descriptor_words.argtypes = [i32, i32]
         L.oracle_set_threads.argtypes = [i32]
+        # the process's CPU share, at most 16 threads (a GPU box's share per GPU)
+        L.oracle_set_threads(min(16, len(os.sched_getaffinity(0))))
         _lib = L
     return _lib
 
