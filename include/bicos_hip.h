@@ -130,6 +130,29 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
                     const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
                     void* stream);
 
+/* bicos_hip_match without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * semantics and results as bicos_hip_match; the caller still passes the
+ * stacks and outputs. The call's scratch (descriptor planes, the int16
+ * disparity before the subpixel stage, reverse-search columns, the MOMENTS
+ * status, packed frames for stacks that are not one uniformly spaced block)
+ * comes from a static per-GPU array of bicos_hip_noalloc_scratch_bytes()
+ * bytes (96 MiB; 1920x1200 with 128-bit descriptors needs about 80 MB),
+ * held by every GPU the library runs on until the process ends; a call
+ * needing more is rejected with BICOS_ERR_INVALID_ARGUMENT before anything
+ * is launched. All calls on one GPU share that array: calls that overlap on
+ * one GPU (on different streams, or from several threads) race and must be
+ * ordered by the caller. Calls no hipMalloc/hipFree; the first call on a
+ * device looks up the array's address there once (hipGetSymbolAddress).
+ * Per-kernel timing (bicos_hip_timing_enable) creates events when it is on. */
+int bicos_hip_match_noalloc(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                            const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
+                            void* stream);
+
+/* Size in bytes of the per-GPU scratch array of bicos_hip_match_noalloc. */
+size_t bicos_hip_noalloc_scratch_bytes(void);
+
 /* Thread-local message of the last failed call on this thread. */
 const char* bicos_hip_last_error(void);
 

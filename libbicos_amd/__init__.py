@@ -83,6 +83,8 @@ def _load():
     L.bicos_hip_corrmap_type.argtypes = [P(_Config)]
     L.bicos_hip_descriptor_bits.argtypes = [u32, i32]
     L.bicos_hip_match.argtypes = [P(_Image), P(_Image), u32, P(_Config), P(_Image), P(_Image), vp]
+    L.bicos_hip_match_noalloc.argtypes = [P(_Image), P(_Image), u32, P(_Config), P(_Image), P(_Image), vp]
+    L.bicos_hip_noalloc_scratch_bytes.restype = C.c_size_t
     L.bicos_hip_transform.argtypes = [P(_Image), u32, i32, i32, vp, vp]
     L.bicos_hip_search.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp, vp]
     L.bicos_hip_agree.argtypes = [vp, P(_Image), P(_Image), u32, f32, i32, f32, i32, vp, vp]
@@ -124,7 +126,8 @@ _lib = _load()
 
 EXPORTED_SYMBOLS = (
     "bicos_hip_config_default", "bicos_hip_disparity_type", "bicos_hip_corrmap_type",
-    "bicos_hip_descriptor_bits", "bicos_hip_match", "bicos_hip_last_error",
+    "bicos_hip_descriptor_bits", "bicos_hip_match", "bicos_hip_match_noalloc",
+    "bicos_hip_noalloc_scratch_bytes", "bicos_hip_last_error",
     "bicos_hip_release_workspace", "bicos_hip_set_search_impl", "bicos_hip_search_chunk_cols",
     "bicos_hip_set_subpixel_impl",
     "bicos_hip_transform", "bicos_hip_search", "bicos_hip_agree",
@@ -298,10 +301,13 @@ def _stream_ptr(stream):
 
 
 def match_device(stack0, stack1, cfg: Optional[Config] = None, disparity=None, corrmap=None,
-                 want_corrmap: bool = False, stream=None):
+                 want_corrmap: bool = False, stream=None, noalloc: bool = False):
     """BICOS::match on device tensors. stack0/stack1: (n, rows, cols) uint8/uint16
     CUDA tensors (or sequences of 2-D tensors). Returns (disparity, corrmap|None)
-    as CUDA tensors; preallocated outputs of the right dtype are reused."""
+    as CUDA tensors; preallocated outputs of the right dtype are reused.
+    noalloc: the library allocates nothing for the call
+    (bicos_hip_match_noalloc: its static per-GPU scratch; calls that overlap
+    on one GPU race)."""
     torch = _torch()
     cfg = cfg or Config()
     c = cfg._c()
@@ -324,8 +330,9 @@ def match_device(stack0, stack1, cfg: Optional[Config] = None, disparity=None, c
     else:
         corrmap = None
     dimg = _img_of_tensor2d(disparity)
-    _check(_lib.bicos_hip_match(s0, s1, n, C.byref(c), C.byref(dimg),
-                                C.byref(cimg) if cimg is not None else None, _stream_ptr(stream)))
+    entry = _lib.bicos_hip_match_noalloc if noalloc else _lib.bicos_hip_match
+    _check(entry(s0, s1, n, C.byref(c), C.byref(dimg), C.byref(cimg) if cimg is not None else None,
+                 _stream_ptr(stream)))
     return disparity, corrmap
 
 

@@ -725,42 +725,50 @@ int with_stacks(const FrameTable* f, const StackInfo* info, int count, uint32_t 
 
 }  // namespace bicos_hip
 
-/* ================================================================== C ABI */
+namespace bicos_hip {
+namespace {
 
-extern "C" {
+/* ------------------------------------------------- no-allocation scratch */
 
-void bicos_hip_config_default(bicos_config* c) {
-    // include/common.hpp:73-82, include/common.hpp:66-69
-    memset(c, 0, sizeof(*c));
-    c->has_nxcorr_threshold = 1;
-    c->nxcorr_threshold = 0.5f;
-    c->mode = BICOS_MODE_LIMITED;
-    c->precision = BICOS_PRECISION_SINGLE;
-    c->variant = BICOS_VARIANT_NO_DUPLICATES;
-    c->max_lr_diff = 1;
-    c->no_dupes = 0;
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)96 << 20;  // 96 MiB per GPU
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+// Scratch of bicos_hip_match_noalloc: a file-scope device array, one copy
+// per GPU (held while the process lives). In host code the array's name is
+// only a stand-in; its device address on each GPU comes from
+// hipGetSymbolAddress with that GPU current, taken once per device and kept
+// in g_noalloc_ptr. Calls on one GPU share it (overlapping calls race); no
+// kernel of the plain path needs it cleared (every byte a call reads was
+// written earlier in the same call).
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch(int dev, void** out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(BICOS_ERR_INVALID_ARGUMENT, "device index beyond the no-allocation table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(BICOS_ERR_HIP, "no device address for the no-allocation scratch");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return BICOS_OK;
 }
 
-int bicos_hip_disparity_type(const bicos_config* c) {
-    return (c->has_nxcorr_threshold && c->has_subpixel_step) ? BICOS_32F : BICOS_16S;
-}
+}  // namespace
 
-int bicos_hip_corrmap_type(const bicos_config* c) {
-    if (!c->has_nxcorr_threshold) return -1;
-    return c->precision == BICOS_PRECISION_DOUBLE ? BICOS_64F : BICOS_32F;
-}
-
-int bicos_hip_descriptor_bits(uint32_t n, int32_t mode) {
-    if (n < 2) return -1;
-    const int w = words_for_bits(required_bits(n, mode));
-    return w ? 32 * w : -1;
-}
-
-const char* bicos_hip_last_error(void) { return g_error.c_str(); }
-
-int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
-                    const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
-                    void* stream) {
+// BICOS::match (src/impl/cuda.cu.in:233-292). The scratch (descriptor
+// planes, the int16 disparity before the subpixel stage, the reverse-search
+// columns, packed frames, the MOMENTS status) comes from the per-device
+// workspace, or with noalloc from the static scratch array (no allocation;
+// calls needing more than NOALLOC_SCRATCH_BYTES are rejected).
+int match_common(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                 const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
+                 void* stream, bool noalloc) {
     if (!cfg) return fail(BICOS_ERR_INVALID_ARGUMENT, "null config");
     hipStream_t s = (hipStream_t)stream;
     StackInfo i0, i1;
@@ -800,11 +808,21 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
     const bool moments = subpix && cfg->precision == BICOS_PRECISION_MOMENTS;
     const size_t mom_bytes = moments ? MomentsScratch::bytes(npx) : 0;
 
+    const size_t need = 2 * desc_bytes + raw_bytes + rev_bytes + pk0 + pk1 + mom_bytes;
     int dev;
     if ((rc = current_device(&dev))) return rc;
-    std::lock_guard<std::mutex> lk(g_ws_mu);
+    std::unique_lock<std::mutex> lk(g_ws_mu, std::defer_lock);
     void* ws;
-    if ((rc = ws_acquire(dev, 2 * desc_bytes + raw_bytes + rev_bytes + pk0 + pk1 + mom_bytes, s, &ws))) return rc;
+    if (noalloc) {
+        if (need > NOALLOC_SCRATCH_BYTES)
+            return fail(BICOS_ERR_INVALID_ARGUMENT,
+                        "this match needs " + std::to_string(need) + " bytes of scratch; the "
+                        "no-allocation entry point has " + std::to_string(NOALLOC_SCRATCH_BYTES));
+        if ((rc = noalloc_scratch(dev, &ws))) return rc;
+    } else {
+        lk.lock();
+        if ((rc = ws_acquire(dev, need, s, &ws))) return rc;
+    }
     uint8_t* p = (uint8_t*)ws;
     uint32_t* desc0 = (uint32_t*)p;
     uint32_t* desc1 = (uint32_t*)(p + desc_bytes);
@@ -844,9 +862,61 @@ int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32
                          cfg->has_min_variance, mv_n, cfg->precision, cm, cstep, s);
     };
     rc = body();
+    if (noalloc) return rc;
     int rc2 = ws_release(dev, s);
     return rc ? rc : rc2;
 }
+
+}  // namespace bicos_hip
+
+using namespace bicos_hip;
+
+/* ================================================================== C ABI */
+
+extern "C" {
+
+void bicos_hip_config_default(bicos_config* c) {
+    // include/common.hpp:73-82, include/common.hpp:66-69
+    memset(c, 0, sizeof(*c));
+    c->has_nxcorr_threshold = 1;
+    c->nxcorr_threshold = 0.5f;
+    c->mode = BICOS_MODE_LIMITED;
+    c->precision = BICOS_PRECISION_SINGLE;
+    c->variant = BICOS_VARIANT_NO_DUPLICATES;
+    c->max_lr_diff = 1;
+    c->no_dupes = 0;
+}
+
+int bicos_hip_disparity_type(const bicos_config* c) {
+    return (c->has_nxcorr_threshold && c->has_subpixel_step) ? BICOS_32F : BICOS_16S;
+}
+
+int bicos_hip_corrmap_type(const bicos_config* c) {
+    if (!c->has_nxcorr_threshold) return -1;
+    return c->precision == BICOS_PRECISION_DOUBLE ? BICOS_64F : BICOS_32F;
+}
+
+int bicos_hip_descriptor_bits(uint32_t n, int32_t mode) {
+    if (n < 2) return -1;
+    const int w = words_for_bits(required_bits(n, mode));
+    return w ? 32 * w : -1;
+}
+
+const char* bicos_hip_last_error(void) { return g_error.c_str(); }
+
+int bicos_hip_match(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                    const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
+                    void* stream) {
+    return match_common(stack0, stack1, n, cfg, disparity, corrmap, stream, false);
+}
+
+int bicos_hip_match_noalloc(const bicos_image* stack0, const bicos_image* stack1, uint32_t n,
+                            const bicos_config* cfg, bicos_image* disparity, bicos_image* corrmap,
+                            void* stream) {
+    return match_common(stack0, stack1, n, cfg, disparity, corrmap, stream, true);
+}
+
+size_t bicos_hip_noalloc_scratch_bytes(void) { return NOALLOC_SCRATCH_BYTES; }
 
 int bicos_hip_set_search_impl(int32_t impl) {
     if (impl != BICOS_SEARCH_MFMA && impl != BICOS_SEARCH_VALU && impl != BICOS_SEARCH_MFMA_I8)
