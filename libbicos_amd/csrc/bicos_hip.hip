@@ -198,20 +198,26 @@ int run_transform(const Stack& f0, const Stack* f1, uint32_t n, int mode, int W,
     if (vec) {
         const dim3 grid((W + 256 * TPX - 1) / (256 * TPX), H, views), block(256);
         if (mode == BICOS_MODE_LIMITED) {
+            // the deepest stack of each width (and 64 frames) with every
+            // frame guard resolved at compile time
+#define TL(WD, NM, PXV, NEV, G) LAUNCH("transform", (k_transform_limited_v4<TIn, WD, NM, PXV, NEV>), G, block, 0, s, f0, g1, (int)n, W, words, out0, out1)
+            const dim3 grid2((W + 511) / 512, H, views);
             switch (words_for_bits(required_bits(n, mode))) {
-                case 1: LAUNCH("transform", (k_transform_limited_v4<TIn, 1, 9>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
-                case 2: LAUNCH("transform", (k_transform_limited_v4<TIn, 2, 17>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
-                case 4: LAUNCH("transform", (k_transform_limited_v4<TIn, 4, 33>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1); break;
+                case 1: if (n == 9) TL(1, 9, TPX, 9, grid); else TL(1, 9, TPX, 0, grid); break;
+                case 2: if (n == 17) TL(2, 17, TPX, 17, grid); else TL(2, 17, TPX, 0, grid); break;
+                case 4: if (n == 33) TL(4, 33, TPX, 33, grid); else TL(4, 33, TPX, 0, grid); break;
                 case 8:
                     if constexpr (sizeof(TIn) == 2) {  // 2 pixels per lane (register budget of 65 frames)
-                        const dim3 grid2((W + 511) / 512, H, views);
-                        LAUNCH("transform", (k_transform_limited_v4<TIn, 8, 65, 2>), grid2, block, 0, s, f0, g1, (int)n, W, words, out0, out1);
+                        if (n == 64) TL(8, 65, 2, 64, grid2);
+                        else if (n == 65) TL(8, 65, 2, 65, grid2);
+                        else TL(8, 65, 2, 0, grid2);
                     } else {
-                        LAUNCH("transform", (k_transform_limited_v4<TIn, 8, 65>), grid, block, 0, s, f0, g1, (int)n, W, words, out0, out1);
+                        TL(8, 65, TPX, 0, grid);
                     }
                     break;
                 default: return fail(BICOS_ERR_INVALID_ARGUMENT, "unsupported stack depth");
             }
+#undef TL
         } else {
             switch (n) {
 #define TF(N) case N: LAUNCH("transform", (k_transform_full_v4<TIn, N>), grid, block, 0, s, f0, g1, W, words, out0, out1); break;

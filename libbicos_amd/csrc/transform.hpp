@@ -238,14 +238,37 @@ __device__ __forceinline__ void store_px(uint32_t* __restrict__ out, int W, int 
             store_desc(out + ((size_t)row * W + col + k) * out_words, w[k], out_words);
 }
 
-// limited mode, runtime n <= NMAX (descriptor_transform.hpp:31-73); PX
-// pixels per lane (4, or 2 for deep 16-bit stacks, where holding 4 pixels'
-// frames would cost the occupancy that hides the memory latency)
-template<typename TIn, int WORDS, int NMAX, int PX = TPX>
+// Sample k of the PX packed raw samples of one frame
+template<typename TIn, int PX>
+__device__ __forceinline__ int raw_sample(const uint32_t (&r)[PX * sizeof(TIn) / 4], int k) {
+    constexpr int PER = 4 / sizeof(TIn);
+    constexpr uint32_t MASK = sizeof(TIn) == 2 ? 0xFFFFu : 0xFFu;
+    return (int)((r[k / PER] >> (8 * sizeof(TIn) * (k % PER))) & MASK);
+}
+
+// shift the comparison x < y of integers into the low end of g
+__device__ __forceinline__ uint32_t shin_i(uint32_t g, int x, int y) {
+    return __builtin_amdgcn_alignbit(g, (uint32_t)(x - y), 31);
+}
+
+// limited mode (descriptor_transform.hpp:31-73); n = NE when NE > 0 (every
+// frame guard resolved at compile time), else runtime n <= NMAX. PX pixels
+// per lane (4, or 2 for deep 16-bit stacks, where holding 4 pixels' frames
+// would cost the occupancy that hides the memory latency).
+//
+// All comparisons are integer: samples and pair sums are small integers,
+// and for an integer p and the reference's FP32 mean av >= 0,
+// p < av <=> p < ceil(av). av itself is float(sum) / n with the sum formed
+// exactly (below 2^24 for n <= 65, as the reference's sequential FP32 sum
+// is). Each comparison x < y is the sign bit of x - y.
+template<typename TIn, int WORDS, int NMAX, int PX = TPX, int NE = 0>
 __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, const Stack f1,
-                                                              int n, int W, int out_words,
+                                                              int n_rt, int W, int out_words,
                                                               uint32_t* __restrict__ out0,
                                                               uint32_t* __restrict__ out1) {
+    static_assert(NE == 0 || NE <= NMAX, "exact depth beyond NMAX");
+    constexpr int RW = PX * sizeof(TIn) / 4;
+    const int n = NE > 0 ? NE : n_rt;
     const int col = (blockIdx.x * 256 + threadIdx.x) * PX, row = blockIdx.y;
     if (col >= W)
         return;
@@ -253,25 +276,24 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
     uint32_t* __restrict__ out = blockIdx.z ? out1 : out0;
 
     // All frames are loaded once and held as raw samples (packed, two per
-    // register for 16-bit input): the mean first, then the comparisons over
-    // a sliding window of three unpacked frames. Half the registers of
-    // holding floats, so more waves hide the memory latency.
-    uint32_t raw[NMAX][PX * sizeof(TIn) / 4];
-    float av[PX];
+    // register for 16-bit input); the comparisons read them in place.
+    uint32_t raw[NMAX][RW];
+    int sum[PX];
 #pragma unroll
-    for (int k = 0; k < PX; ++k) av[k] = 0.0f;
+    for (int k = 0; k < PX; ++k) sum[k] = 0;
 #pragma unroll
     for (int t = 0; t < NMAX; ++t) {
+#pragma unroll
+        for (int j = 0; j < RW; ++j) raw[t][j] = 0u;
         if (t < n) {
             load_raw<TIn, PX>(f, t, row, col, W, raw[t]);
-            float v[PX];
-            unpack<TIn, PX>(raw[t], v);
 #pragma unroll
-            for (int k = 0; k < PX; ++k) av[k] += v[k];
+            for (int k = 0; k < PX; ++k) sum[k] += raw_sample<TIn, PX>(raw[t], k);
         }
     }
+    int avc[PX];  // ceil of the reference's FP32 mean
 #pragma unroll
-    for (int k = 0; k < PX; ++k) av[k] /= (float)n;
+    for (int k = 0; k < PX; ++k) avc[k] = (int)__builtin_ceilf((float)sum[k] / (float)n);
 
     uint32_t w[PX][WORDS];
 #pragma unroll
@@ -279,31 +301,31 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
 #pragma unroll
         for (int j = 0; j < WORDS; ++j) w[k][j] = 0u;
 
-    float pa[PX], pb[PX], pc[PX];  // frames t, t + 1, t + 2
-    unpack<TIn, PX>(raw[0], pa);
-    unpack<TIn, PX>(raw[1], pb);
-    float prev0[PX], prev1[PX];
+    int prev0[PX], prev1[PX];  // pair sums S(t - 2) by parity of t
+    int pa[PX], pb[PX], pc[PX];  // samples of frames t, t + 1, t + 2 (unpacked as they are needed)
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+        pa[k] = raw_sample<TIn, PX>(raw[0], k);
+        pb[k] = raw_sample<TIn, PX>(raw[1], k);
+    }
 #pragma unroll
     for (int t = 0; t < NMAX - 2; ++t) {
         if (t < n - 2) {
             const int pos = t < 2 ? 3 * t : 4 * t - 2;
-            unpack<TIn, PX>(raw[t + 2], pc);
 #pragma unroll
             for (int k = 0; k < PX; ++k) {
-                const float a = pa[k], b = pb[k], c = pc[k];
-                const float cur = a + b;
+                pc[k] = raw_sample<TIn, PX>(raw[t + 2], k);
+                const int a = pa[k], b = pb[k], c = pc[k];
+                const int cur = a + b;
                 uint32_t g = 0;
-                if (t >= 2) g = shin(g, (t & 1) ? prev1[k] : prev0[k], cur);
-                g = shin(g, a, av[k]);
-                g = shin(g, a, c);
-                g = shin(g, a, b);
+                if (t >= 2) g = shin_i(g, (t & 1) ? prev1[k] : prev0[k], cur);
+                g = shin_i(g, a, avc[k]);
+                g = shin_i(g, a, c);
+                g = shin_i(g, a, b);
                 put_group(w[k], pos, t >= 2 ? 4 : 3, g);
                 if (t & 1) prev1[k] = cur; else prev0[k] = cur;
-            }
-#pragma unroll
-            for (int k = 0; k < PX; ++k) {
-                pa[k] = pb[k];
-                pb[k] = pc[k];
+                pa[k] = b;
+                pb[k] = c;
             }
         }
     }
@@ -314,13 +336,13 @@ __global__ __launch_bounds__(256) void k_transform_limited_v4(const Stack f0, co
     const int wi = P >> 5, sh = P & 31;
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
-        const float a = pa[k], b = pb[k];
-        const float prevf = n < 4 ? -1.0f : (((n - 2) & 1) ? prev1[k] : prev0[k]);
+        const int a = pa[k], b = pb[k];
+        const int prevf = n < 4 ? -1 : (((n - 2) & 1) ? prev1[k] : prev0[k]);
         uint32_t g = 0;
-        g = shin(g, prevf, a + b);
-        g = shin(g, b, av[k]);
-        g = shin(g, a, av[k]);
-        g = shin(g, a, b);
+        g = shin_i(g, prevf, a + b);
+        g = shin_i(g, b, avc[k]);
+        g = shin_i(g, a, avc[k]);
+        g = shin_i(g, a, b);
 #pragma unroll
         for (int j = 0; j < WORDS; ++j) {
             if (j == wi) w[k][j] |= g << sh;
