@@ -53,19 +53,25 @@ enum {
 /* TransformMode (include/common.hpp:58) */
 enum { BICOS_MODE_LIMITED = 0, BICOS_MODE_FULL = 1 };
 
-/* Precision (include/common.hpp:59-61). HALF_MFMA and MOMENTS have no
- * reference counterpart and concern the subpixel stage (the single-point
- * agreement stays the exact FP32 one); neither is bit-exact (see the
- * precision tests):
- *   HALF_MFMA  grid points scored from FP16 matrix-core cross products and
- *              FP64 means/variances, winners rescored in FP32;
- *   MOMENTS    grid points scored in closed form from exact (FP64) sample
- *              moments, winners rescored in FP32. */
+/* Precision (include/common.hpp:59-61). HALF_MFMA, MOMENTS and
+ * MOMENTS_FAST have no reference counterpart and concern the subpixel stage
+ * (the single-point agreement stays the exact FP32 one); none is bit-exact
+ * (see the precision tests):
+ *   HALF_MFMA     grid points scored from FP16 matrix-core cross products
+ *                 and FP64 means/variances, winners rescored in FP32;
+ *   MOMENTS       grid points scored in closed form from exact (FP64) sample
+ *                 moments; accept/reject is the reference's (an error bound
+ *                 sends undecided pixels to the exact FP32 stage);
+ *   MOMENTS_FAST  the same closed form from exact moments, decided on its
+ *                 own FP32 values (no exact stage): pixels within about 1e-4
+ *                 of the threshold can be decided the other way. Needs at most 64
+ *                 grid points and no corrmap (else the exact FP32 stage runs). */
 enum {
     BICOS_PRECISION_SINGLE = 0,
     BICOS_PRECISION_DOUBLE = 1,
     BICOS_PRECISION_HALF_MFMA = 2,
-    BICOS_PRECISION_MOMENTS = 3
+    BICOS_PRECISION_MOMENTS = 3,
+    BICOS_PRECISION_MOMENTS_FAST = 4
 };
 
 /* SearchVariant (include/common.hpp:63-71) */

@@ -165,13 +165,16 @@ class TransformMode:
 
 
 class Precision:
-    """include/common.hpp:59-61; HalfMFMA and Moments have no reference
-    counterpart (subpixel candidates scored from FP16 matrix-core products,
-    resp. in closed form from exact moments; winners rescored in FP32)."""
+    """include/common.hpp:59-61; HalfMFMA, Moments and MomentsFast have no
+    reference counterpart (subpixel candidates scored from FP16 matrix-core
+    products, winners rescored in FP32; resp. in closed form from exact
+    moments, with the reference's accept/reject decisions (Moments) or
+    decided on the closed form's own values (MomentsFast))."""
     Single = 0
     Double = 1
     HalfMFMA = 2
     Moments = 3
+    MomentsFast = 4
 
 
 class Variant:

@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void k_agree_list(int16_t* __restrict__ disp, 
     for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
         int row, col;
         list_entry(l, i, row, col);
+        if (disp[(size_t)row * dstep + col] == LISTED_SAME16) continue;  // keeps the windowed pass's outputs
         agree_px<TIn, TPrec, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, has_minvar,
                                             minvar, corrmap, cstep);
     }
@@ -290,6 +291,7 @@ __global__ __launch_bounds__(256) void k_agree_subpixel_list(
     for (int i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
         int row, col;
         list_entry(l, i, row, col);
+        if (disp[(size_t)row * dstep + col] == LISTED_SAME16) continue;  // keeps the windowed pass's outputs
         agree_subpixel_px<TIn, TPrec, NMAX, N_EXACT>(row, col, disp, dstep, f0, f1, n_rt, W, thr, step,
                                                      has_minvar, minvar, out, ostep, corrmap, cstep);
     }

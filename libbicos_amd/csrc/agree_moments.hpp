@@ -220,13 +220,6 @@ __device__ void exact_pending_wave(uint64_t pend, int row, int col, int d, const
     }
 }
 
-// The grid points x_k of the subpixel scan (x_0 = -1, x_k = x_(k-1) + step
-// in FP32, agree.hpp:160), formed on the host the way the reference's loop
-// forms them; used where npts <= 64.
-struct GridX {
-    float x[64];
-};
-
 // Pixels the exact candidate stage takes per round of one wave (each brings
 // its candidate grid points, one lane per point, 64 lanes per round); fewer
 // for deep stacks, so that a workgroup's staging stays near 32 KiB of LDS.
@@ -719,7 +712,9 @@ __global__ __launch_bounds__(256) void k_agree_subpixel_moments_list(
         const int i = wbase + lane;
         int row, col;
         list_entry(l, min(i, total - 1), row, col);
-        agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(i < total, row, col, disp, dstep, f0, f1, n_rt, W,
+        // LISTED_SAME16: keeps the windowed pass's outputs (inactive lane)
+        const bool act = i < total && disp[(size_t)row * dstep + col] != LISTED_SAME16;
+        agree_subpixel_moments_px<TIn, NMAX, N_EXACT>(act, row, col, disp, dstep, f0, f1, n_rt, W,
                                                       thr, step, npts, has_minvar, minvar, out, ostep,
                                                       status, xs[threadIdx.x >> 6]);
     }

@@ -22,6 +22,7 @@
 #include "../../include/bicos_hip.h"
 #include "agree.hpp"
 #include "agree_half.hpp"
+#include "agree_fast.hpp"
 #include "agree_moments.hpp"
 #include "search.hpp"
 #include "search_mfma.hpp"
@@ -102,7 +103,8 @@ int ws_release(int dev, hipStream_t s, int slot) {
 
 bool precision_known(int p) {
     return p == BICOS_PRECISION_SINGLE || p == BICOS_PRECISION_DOUBLE ||
-           p == BICOS_PRECISION_HALF_MFMA || p == BICOS_PRECISION_MOMENTS;
+           p == BICOS_PRECISION_HALF_MFMA || p == BICOS_PRECISION_MOMENTS ||
+           p == BICOS_PRECISION_MOMENTS_FAST;
 }
 
 int elem_size(int type) {
@@ -397,20 +399,6 @@ int search_flags(const uint32_t* d0, const uint32_t* d1, int H, int W, int words
     return run_search(d0, d1, H, W, words, nodupes, SEARCH_DISP, nullptr, 0, out, ostep, s);
 }
 
-// stack depth buckets of the NCC kernels (registers per lane scale with NMAX)
-template<typename F>
-int with_nmax(uint32_t n, F&& f) {
-    if (n == 12) return f(std::integral_constant<int, 12>{}, std::integral_constant<int, 12>{});
-    if (n == 33) return f(std::integral_constant<int, 33>{}, std::integral_constant<int, 33>{});
-    if (n == 64) return f(std::integral_constant<int, 64>{}, std::integral_constant<int, 64>{});
-    if (n <= 8) return f(std::integral_constant<int, 8>{}, std::integral_constant<int, 0>{});
-    if (n <= 16) return f(std::integral_constant<int, 16>{}, std::integral_constant<int, 0>{});
-    if (n <= 24) return f(std::integral_constant<int, 24>{}, std::integral_constant<int, 0>{});
-    if (n <= 33) return f(std::integral_constant<int, 33>{}, std::integral_constant<int, 0>{});
-    if (n <= 48) return f(std::integral_constant<int, 48>{}, std::integral_constant<int, 0>{});
-    return f(std::integral_constant<int, 65>{}, std::integral_constant<int, 0>{});
-}
-
 template<typename TIn, typename TPrec>
 int run_agree_t(int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                 uint32_t n, int W, int H, float thr, int has_mv, float mv, TPrec* corr,
@@ -472,19 +460,46 @@ template<typename TIn, typename TPrec>
 int run_subpixel_pruned_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
                           uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
                           float* out, int64_t ostep, hipStream_t s) {
-    GridX gx;
+    const GridX gx = grid_x(step);
     const int npts = subpixel_points(step);
-    float x = -1.0f;  // the reference's running sum (agree.hpp:160)
-    for (int k = 0; k < 64; ++k) {
-        gx.x[k] = k < npts ? x : 0.0f;
-        x += step;
-    }
     return with_nmax(n, [&](auto NM, auto NE) -> int {
         constexpr int NMAX = decltype(NM)::value, NEX = decltype(NE)::value;
         LAUNCH("agree_subpixel_pruned", (k_agree_subpixel_pruned<TIn, TPrec, NMAX, NEX>), dim3((W + 255) / 256, H), dim3(256), 0, s,
                disp, dstep, f0, f1, (int)n, W, thr, step, npts, has_mv, mv, out, ostep, gx);
         return BICOS_OK;
     });
+}
+
+template<typename TIn>
+int run_subpixel_fast_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                        uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
+                        float* out, int64_t ostep, hipStream_t s) {
+    const GridX gx = grid_x(step);
+    const int npts = subpixel_points(step);
+    LAUNCH("agree_subpixel_fast", k_subpixel_fast<TIn>, dim3((W + 255) / 256, H), dim3(256), 0, s,
+           disp, dstep, f0, f1, (int)n, W, thr, npts, gx, has_mv, mv, out, ostep);
+    return BICOS_OK;
+}
+
+template<typename TIn>
+int run_subpixel_fast_list_t(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                             uint32_t n, int W, int H, float thr, float step, int has_mv, float mv,
+                             float* out, int64_t ostep, const int16_t* list, const int* row_count,
+                             hipStream_t s) {
+    const GridX gx = grid_x(step);
+    const int npts = subpixel_points(step);
+    LAUNCH("agree_subpixel_fast_list", k_subpixel_fast_list<TIn>, dim3(H), dim3(256), 0, s,
+           disp, dstep, f0, f1, (int)n, W, thr, npts, gx, has_mv, mv, out, ostep, list, row_count);
+    return BICOS_OK;
+}
+
+int run_subpixel_fast_list(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                           int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                           float mv_n, float* out, int64_t ostep, const int16_t* list,
+                           const int* row_count, hipStream_t s) {
+    return depth == BICOS_8U
+        ? run_subpixel_fast_list_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, list, row_count, s)
+        : run_subpixel_fast_list_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, list, row_count, s);
 }
 
 template<typename TIn>
@@ -570,8 +585,15 @@ int run_subpixel(const int16_t* disp, int64_t dstep, const Stack& f0, const Stac
         return run_moments_only(disp, dstep, f0, f1, depth, n, W, H, thr, step, has_mv, mv_n, out,
                                 ostep, ms, s);
     }
-    if (precision == BICOS_PRECISION_MOMENTS)  // a corrmap is written: the exact stage throughout
-        precision = BICOS_PRECISION_SINGLE;
+    if (precision == BICOS_PRECISION_MOMENTS_FAST && !corr) {
+        const int npts = subpixel_points(step);
+        if (npts >= 1 && npts <= 64)
+            return depth == BICOS_8U
+                ? run_subpixel_fast_t<uint8_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s)
+                : run_subpixel_fast_t<uint16_t>(disp, dstep, f0, f1, n, W, H, thr, step, has_mv, mv_n, out, ostep, s);
+    }
+    if (precision == BICOS_PRECISION_MOMENTS || precision == BICOS_PRECISION_MOMENTS_FAST)
+        precision = BICOS_PRECISION_SINGLE;  // a corrmap is written: the exact stage throughout
     if ((precision == BICOS_PRECISION_SINGLE || precision == BICOS_PRECISION_DOUBLE) && !corr &&
         g_subpixel_impl == BICOS_SUBPIXEL_PRUNED) {
         const int npts = subpixel_points(step);

@@ -29,6 +29,24 @@ struct Stack {
 
 constexpr int16_t INVALID16 = -32768;  // INVALID_DISP<int16_t> (include/common.hpp:34-37)
 
+// A listed pixel's full-row match replaces its windowed match in dst. When
+// the two are the same valid match, the NCC stage has already scored it
+// (the pixel is listed because that match was rejected): LISTED_SAME16 is
+// stored instead, and the listed NCC stage leaves the pixel's outputs as the
+// windowed pass wrote them. (-32767 is no disparity: |d| < 32767 for rows of
+// at most 32767 columns.)
+constexpr int16_t LISTED_SAME16 = -32767;
+__device__ __forceinline__ void set_listed(int16_t* dst, int16_t v) {
+    *dst = (v == *dst && v != INVALID16) ? LISTED_SAME16 : v;
+}
+
+// The grid points x_k of the subpixel scan (x_0 = -1, x_k = x_(k-1) + step
+// in FP32, agree.hpp:160), formed on the host the way the reference's loop
+// forms them (grid_x); passed by value where npts <= 64.
+struct GridX {
+    float x[64];
+};
+
 template<typename T>
 __device__ __forceinline__ const T* frame_row(const Stack& f, int t, int row) {
     return (const T*)(f.base + (int64_t)t * f.fstride + (int64_t)row * f.step);

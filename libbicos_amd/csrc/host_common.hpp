@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bicos_hip.h"
@@ -93,7 +94,43 @@ size_t packed_bytes(const FrameTable& f, uint32_t n, const StackInfo& i);
 int make_stack(const FrameTable& f, uint32_t n, const StackInfo& i, void* scratch, Stack* out,
                hipStream_t s);
 int check_step(float step);
+// Raises a kernel's dynamic LDS limit once where `lds` exceeds the default 64 KiB
+int allow_dynamic_lds(const void* kernel, size_t lds);
 int subpixel_points(float step);
+
+// stack depth buckets of the NCC kernels (registers per lane scale with NMAX)
+template<typename F>
+int with_nmax(uint32_t n, F&& f) {
+    if (n == 12) return f(std::integral_constant<int, 12>{}, std::integral_constant<int, 12>{});
+    if (n == 33) return f(std::integral_constant<int, 33>{}, std::integral_constant<int, 33>{});
+    if (n == 64) return f(std::integral_constant<int, 64>{}, std::integral_constant<int, 64>{});
+    if (n <= 8) return f(std::integral_constant<int, 8>{}, std::integral_constant<int, 0>{});
+    if (n <= 16) return f(std::integral_constant<int, 16>{}, std::integral_constant<int, 0>{});
+    if (n <= 24) return f(std::integral_constant<int, 24>{}, std::integral_constant<int, 0>{});
+    if (n <= 33) return f(std::integral_constant<int, 33>{}, std::integral_constant<int, 0>{});
+    if (n <= 48) return f(std::integral_constant<int, 48>{}, std::integral_constant<int, 0>{});
+    return f(std::integral_constant<int, 65>{}, std::integral_constant<int, 0>{});
+}
+
+// The grid points x_k of the subpixel scan as the reference's loop forms them
+// (x_0 = -1, x_k = x_(k-1) + step in FP32, agree.hpp:160); the first 64.
+inline GridX grid_x(float step) {
+    GridX gx;
+    const int npts = subpixel_points(step);
+    float x = -1.0f;
+    for (int k = 0; k < 64; ++k) {
+        gx.x[k] = k < npts ? x : 0.0f;
+        x += step;
+    }
+    return gx;
+}
+
+// MOMENTS_FAST subpixel stage of the listed pixels (row r: row_count[r]
+// columns at list + r * W; needs no row_start)
+int run_subpixel_fast_list(const int16_t* disp, int64_t dstep, const Stack& f0, const Stack& f1,
+                           int depth, uint32_t n, int W, int H, float thr, float step, int has_mv,
+                           float mv_n, float* out, int64_t ostep, const int16_t* list,
+                           const int* row_count, hipStream_t s);
 int check_out(const bicos_image* img, int rows, int cols, int type, const char* what);
 
 /* ------------------------------------------------------ stage dispatch */

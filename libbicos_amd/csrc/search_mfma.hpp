@@ -36,6 +36,7 @@
 
 namespace bicos_hip {
 
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
             if constexpr (MODE == SEARCH_REV)
                 res[(size_t)row * W + i] = best;
             else if (col >= 0)
-                out[(size_t)row * ostep + col] = valid ? (int16_t)(col - best) : INVALID16;
+                set_listed(out + (size_t)row * ostep + col, valid ? (int16_t)(col - best) : INVALID16);
             continue;
         }
         if (g != 0 || col >= W)
@@ -573,7 +574,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
             if constexpr (MODE == SEARCH_REV)
                 res[(size_t)row * W + i] = bst;
             else if (col >= 0)
-                out[(size_t)row * ostep + col] = valid ? (int16_t)(col - best) : INVALID16;
+                set_listed(out + (size_t)row * ostep + col, valid ? (int16_t)(col - best) : INVALID16);
             continue;
         }
         if (g != 0 || col >= W)

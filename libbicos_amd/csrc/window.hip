@@ -395,6 +395,49 @@ int run_fallback_list(const float* disp_f, const int16_t* disp_i, int64_t dstep,
     return run_row_scan(row_count, g.H, s);
 }
 
+// The fused windowed pass of the MOMENTS_FAST pipeline (k_window_core)
+template<typename TIn, int WORDS>
+int run_window_core_w(const uint32_t* d0, const uint32_t* d1, const Grid& g, bool nodupes,
+                      bool consist, int max_lr, const float* center, const int16_t* hw,
+                      const Stack& v0, const Stack& v1, uint32_t n, float thr, float step,
+                      int has_mv, float mv_n, float* out, int64_t ostep, int fallback, int hw_lo,
+                      int16_t* list, int* row_count, int16_t* dwin, hipStream_t s) {
+    const GridX gx = grid_x(step);
+    const int npts = subpixel_points(step);
+    const dim3 grid((g.W + 255) / 256, g.H), block(256);
+    const int fb = fallback != BICOS_FALLBACK_NONE, unsure = fallback == BICOS_FALLBACK_UNSURE;
+    // staged span: at most 1024 columns and 32 KiB of LDS per block
+    const int span = std::min(1024, (int)((32u << 10) / core_lds_bytes(WORDS, consist, 1))) & ~7;
+    const size_t lds = core_lds_bytes(WORDS, consist, span);
+#define WC(ND, CS) do { \
+        int rc_ = allow_dynamic_lds((const void*)k_window_core<TIn, WORDS, ND, CS>, lds); \
+        if (rc_) return rc_; \
+        LAUNCH("window_core", (k_window_core<TIn, WORDS, ND, CS>), grid, block, lds, s, d0, d1, center, hw, g.Hq, g.Wq, max_lr, v0, v1, (int)n, g.W, thr, npts, gx, has_mv, mv_n, out, ostep, fb, unsure, hw_lo, list, row_count, span, dwin); \
+    } while (0)
+    if (!consist) WC(true, false);
+    else if (nodupes) WC(true, true);
+    else WC(false, true);
+#undef WC
+    return BICOS_OK;
+}
+
+template<typename TIn>
+int run_window_core_t(const uint32_t* d0, const uint32_t* d1, const Grid& g, int words, bool nodupes,
+                      bool consist, int max_lr, const float* center, const int16_t* hw,
+                      const Stack& v0, const Stack& v1, uint32_t n, float thr, float step,
+                      int has_mv, float mv_n, float* out, int64_t ostep, int fallback, int hw_lo,
+                      int16_t* list, int* row_count, int16_t* dwin, hipStream_t s) {
+#define WCW(WD) return run_window_core_w<TIn, WD>(d0, d1, g, nodupes, consist, max_lr, center, hw, v0, v1, n, thr, step, has_mv, mv_n, out, ostep, fallback, hw_lo, list, row_count, dwin, s)
+    switch (words) {
+        case 1: WCW(1);
+        case 2: WCW(2);
+        case 4: WCW(4);
+        case 8: WCW(8);
+        default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
+    }
+#undef WCW
+}
+
 }  // namespace
 }  // namespace bicos_hip
 
@@ -655,6 +698,12 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
     uint8_t* pack0 = p;
     uint8_t* pack1 = p + pk0;
 
+    // MOMENTS_FAST without a corrmap: the windowed search, the subpixel stage
+    // and the fallback set in one pass (k_window_core), then the fallback's
+    // full-row search and its subpixel stage on the list
+    const int npts_fast = subpix ? subpixel_points(cfg->subpixel_step) : 0;
+    const bool fast = subpix && cfg->precision == BICOS_PRECISION_MOMENTS_FAST && !corrmap &&
+                      npts_fast >= 1 && npts_fast <= 64;
     auto body = [&]() -> int {
         int r;
         Stack v0, v1;
@@ -666,8 +715,13 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         if ((r = side_stream(dev, &ss))) return r;
         HIP_TRY(hipEventRecord(ss->fork, s));
         HIP_TRY(hipStreamWaitEvent(ss->stream, ss->fork, 0));
-        const int rc_cnn = cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch,
-                                       center, hw, nullptr, ss->stream);
+        int rc_zero = BICOS_OK;
+        if (fast && fallback)  // the fused pass appends to the row lists
+            rc_zero = launch("fallback_list", k_zero_u32, dim3((H + 255) / 256), dim3(256), 0, ss->stream,
+                             (unsigned int*)row_count, H);
+        const int rc_cnn = rc_zero ? rc_zero
+                                   : cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch,
+                                                 center, hw, nullptr, ss->stream);
         // joined in every case: the workspace is released on the caller's stream
         HIP_TRY(hipEventRecord(ss->join, ss->stream));
         r = rc_cnn ? rc_cnn
@@ -676,6 +730,39 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         if (r) return r;
         int16_t* d16 = subpix ? raw : (int16_t*)disparity->data;
         const int64_t d16_step = subpix ? W : disparity->step / 2;
+        if (fast) {
+            const float mv_n = cfg->has_min_variance ? cfg->min_variance * (float)n : 0.0f;
+            float* outf = (float*)disparity->data;
+            const int64_t ostep = disparity->step / 4;
+            const int fbm = fallback ? opts->fallback : BICOS_FALLBACK_NONE;
+            r = i0.depth == BICOS_8U
+                ? run_window_core_t<uint8_t>(desc0, desc1, g, words, nodupes, consistency, max_lr, center, hw, v0, v1, n,
+                                             cfg->nxcorr_threshold, cfg->subpixel_step, cfg->has_min_variance, mv_n,
+                                             outf, ostep, fbm, opts->hw_lo, list, row_count, d16, s)
+                : run_window_core_t<uint16_t>(desc0, desc1, g, words, nodupes, consistency, max_lr, center, hw, v0, v1, n,
+                                              cfg->nxcorr_threshold, cfg->subpixel_step, cfg->has_min_variance, mv_n,
+                                              outf, ostep, fbm, opts->hw_lo, list, row_count, d16, s);
+            if (r || !fallback) return r;
+            if (consistency) {
+                if ((r = run_search_list(desc0, desc1, H, W, words, nodupes, SEARCH_REV, list, row_count,
+                                         fwd, nullptr, 0, s)))
+                    return r;
+                if ((r = run_search_list(desc1, desc0, H, W, words, nodupes, SEARCH_REV, fwd, row_count,
+                                         rev, nullptr, 0, s)))
+                    return r;
+                LAUNCH("lr_combine", k_lr_combine, dim3(H, 1), dim3(256), 0, s,
+                       (const int16_t*)list, (const int*)row_count, W, (const int16_t*)fwd,
+                       (const int16_t*)rev, max_lr, d16, d16_step);
+            } else if ((r = run_search_list(desc0, desc1, H, W, words, nodupes, SEARCH_DISP, list, row_count,
+                                            nullptr, d16, d16_step, s))) {
+                return r;
+            }
+            if (stats)
+                LAUNCH("fallback_list", k_sum_counts, dim3(1), dim3(256), 0, s, (const int*)row_count, H, fb_total);
+            return run_subpixel_fast_list(d16, d16_step, v0, v1, i0.depth, n, W, H, cfg->nxcorr_threshold,
+                                          cfg->subpixel_step, cfg->has_min_variance, mv_n, outf, ostep, list,
+                                          row_count, s);
+        }
         if ((r = run_window_search(desc0, desc1, g, words, nodupes, consistency, max_lr, center, hw,
                                    d16, d16_step, s)))
             return r;

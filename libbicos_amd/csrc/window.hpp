@@ -11,6 +11,7 @@
 #pragma once
 
 #include "agree.hpp"
+#include "agree_fast.hpp"
 #include "agree_moments.hpp"
 #include "cnn.hpp"
 #include "device_common.hpp"
@@ -91,6 +92,196 @@ __global__ __launch_bounds__(256) void k_window_search(const uint32_t* __restric
         }
     }
     out[(size_t)row * ostep + col] = v;
+}
+
+// Two smallest keys of the candidates [lo, hi] of one row, read from `src`
+// (LDS or global memory) at column c - base; columns are 16-byte aligned
+// for WORDS >= 4 (read as uint4).
+template<int WORDS, bool NODUPES>
+__device__ __forceinline__ void scan_keys(const uint32_t (&q)[WORDS], const uint32_t* src, int base,
+                                          int lo, int hi, uint32_t& b1, uint32_t& b2) {
+    for (int c = lo; c <= hi; ++c) {
+        uint32_t d[WORDS];
+        if constexpr (WORDS >= 4) {
+            const uint4* p4 = (const uint4*)src + (size_t)(c - base) * (WORDS / 4);
+#pragma unroll
+            for (int k = 0; k < WORDS / 4; ++k) {
+                const uint4 v = p4[k];
+                d[4 * k] = v.x; d[4 * k + 1] = v.y; d[4 * k + 2] = v.z; d[4 * k + 3] = v.w;
+            }
+        } else {
+            load_desc<WORDS>(d, src + (size_t)(c - base) * WORDS);
+        }
+        uint32_t cost = 0;
+#pragma unroll
+        for (int k = 0; k < WORDS; ++k) cost = __popc(q[k] ^ d[k]) + cost;
+        const uint32_t key = (cost << 16) | (uint32_t)c;
+        if constexpr (NODUPES) b2 = umed3(b1, b2, key);
+        b1 = min(b1, key);
+    }
+}
+
+// The window scan of one pixel over [lo, hi]: columns [base, base + len) of
+// the row are staged in `lds`, the rest is read from `grow` (the row in
+// global memory). Candidates are taken in column order (the first strict
+// minimum needs it): the staged part, then the columns after it.
+template<int WORDS, bool NODUPES>
+__device__ __forceinline__ void scan_staged(const uint32_t (&q)[WORDS], const uint32_t* lds, int base,
+                                            int len, const uint32_t* grow, int lo, int hi,
+                                            uint32_t& b1, uint32_t& b2) {
+    b1 = 0xFFFFFFFFu;
+    b2 = 0xFFFFFFFFu;
+    const int split = base + len;  // lo >= base always
+    if (lo < base) {
+        scan_keys<WORDS, NODUPES>(q, grow, 0, lo, hi, b1, b2);
+        return;
+    }
+    scan_keys<WORDS, NODUPES>(q, lds, base, lo, min(hi, split - 1), b1, b2);
+    scan_keys<WORDS, NODUPES>(q, grow, 0, max(lo, split), hi, b1, b2);
+}
+
+// Copies `total` 16-byte units from global memory to LDS, ld(u) -> uint4
+// and st(u, v), B units per thread in flight at once.
+template<int B, typename Ld, typename St>
+__device__ __forceinline__ void stage_units(int total, Ld&& ld, St&& st) {
+    for (int u0 = (int)threadIdx.x; u0 < total; u0 += 256 * B) {
+        uint4 v[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+            if (u0 + 256 * b < total) v[b] = ld(u0 + 256 * b);
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+            if (u0 + 256 * b < total) st(u0 + 256 * b, v[b]);
+    }
+}
+
+// LDS of k_window_core for a span of `span` staged columns (a multiple of 8)
+__host__ __device__ constexpr size_t core_lds_bytes(int words, bool consist, int span) {
+    return (size_t)span * words * 4 * (consist ? 2 : 1);
+}
+
+// The learned path's windowed pass in one kernel. A block takes 256 pixels
+// of one row and stages in LDS, all loads in flight together, the right
+// descriptors of the columns its windows reach (up to `span` columns from
+// the first window's start, less one, rounded down to 8; windows reaching
+// past them read the rest from global memory) and, with Consistency, the
+// left descriptors of its back-search windows. Then per pixel: the windowed
+// search (k_window_search's semantics), the MOMENTS_FAST subpixel stage on
+// its match, and the fallback set (invalid result; not a pixel whose left
+// variance rejects every match; with `unsure`, only cells given the wide
+// window) appended to the row's list (row_count[row] entries at
+// list + row * W, zeroed beforehand), its windowed match kept in dwin for
+// the full-row search of the list. npts <= 64 grid points.
+template<typename TIn, int WORDS, bool NODUPES, bool CONSIST>
+__global__ __launch_bounds__(256) void k_window_core(
+    const uint32_t* __restrict__ d0, const uint32_t* __restrict__ d1, const float* __restrict__ center_q,
+    const int16_t* __restrict__ hw_q, int Hq, int Wq, int max_lr_diff, const Stack f0, const Stack f1,
+    int n, int W, float thr, int npts, const GridX gx, int has_mv, float minvar,
+    float* __restrict__ out, int64_t ostep, int fallback, int unsure, int hw_lo,
+    int16_t* __restrict__ list, int* __restrict__ row_count, int span, int16_t* __restrict__ dwin) {
+    extern __shared__ uint4 core_lds[];
+    uint32_t* rdesc = (uint32_t*)core_lds;                    // span x WORDS
+    uint32_t* ldesc = rdesc + (size_t)span * WORDS;           // span x WORDS (Consistency)
+    __shared__ int red[4][3], wsum[4], base_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row = blockIdx.y, c0 = blockIdx.x * 256;
+    const int col = min(c0 + tid, W - 1);
+    const bool act = c0 + tid < W;
+
+    const int cq = min(row / CNN_DS, Hq - 1) * Wq + min(col / CNN_DS, Wq - 1);
+    const int k1 = col - (int)__builtin_rintf(center_q[cq]);
+    const int hw = hw_q[cq];
+    const int lo = max(k1 - hw, 0), hi = min(k1 + hw, W - 1);
+    // block extent of the windows and the largest half-width
+    int mlo = act && lo <= hi ? lo : 0x7FFFFFFF, mhi = act && lo <= hi ? hi : -1;
+    int mhw = act ? hw : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        mlo = min(mlo, __shfl_xor(mlo, o));
+        mhi = max(mhi, __shfl_xor(mhi, o));
+        mhw = max(mhw, __shfl_xor(mhw, o));
+    }
+    if (lane == 0) {
+        red[wave][0] = mlo;
+        red[wave][1] = mhi;
+        red[wave][2] = mhw;
+    }
+    __syncthreads();
+    const int rbase = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
+    const int rlast = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
+    // staged right columns [sa, sa + slen): the windows' reach plus one column
+    // each side (the subpixel stage reads col1 - 1 .. col1 + 1)
+    const int sa = rlast >= 0 ? max(rbase - 1, 0) & ~7 : 0;
+    const int slen = rlast >= 0 ? min(min(rlast + 1, W - 1) - sa + 1, span) : 0;
+    const uint32_t* rrow = d1 + (size_t)row * W * WORDS;
+    const uint32_t* lrow = d0 + (size_t)row * W * WORDS;
+    const auto stage_desc = [&](const uint32_t* src, int first, int len, uint32_t* dst) {
+        if constexpr (WORDS >= 4) {
+            stage_units<4>(len * (WORDS / 4), [&](int u) { return ((const uint4*)(src + (size_t)first * WORDS))[u]; },
+                           [&](int u, uint4 v) { ((uint4*)dst)[u] = v; });
+        } else {
+            for (int i = tid; i < len * WORDS; i += 256) dst[i] = src[(size_t)first * WORDS + i];
+        }
+    };
+    stage_desc(rrow, sa, slen, rdesc);
+    int lb = 0, llen = 0;
+    if constexpr (CONSIST) {
+        const int bhw = max(max(red[0][2], red[1][2]), max(red[2][2], red[3][2]));
+        lb = max(c0 - bhw, 0);
+        llen = min(min(c0 + 255 + bhw, W - 1) - lb + 1, span);
+        stage_desc(lrow, lb, llen, ldesc);
+    }
+    __syncthreads();
+
+    uint32_t q[WORDS];
+    load_desc<WORDS>(q, lrow + (size_t)col * WORDS);
+    uint32_t b1, b2;
+    scan_staged<WORDS, NODUPES>(q, rdesc, sa, slen, rrow, lo, hi, b1, b2);
+    int d = INVALID16;
+    if (lo <= hi && !(NODUPES && (b1 >> 16) == (b2 >> 16))) {
+        const int best = (int)(b1 & 0xFFFFu);
+        if constexpr (!CONSIST) {
+            d = col - best;
+        } else {
+            uint32_t r[WORDS];
+            load_desc<WORDS>(r, rrow + (size_t)best * WORDS);
+            uint32_t r1, r2;
+            const int blo = max(col - hw, 0), bhi = min(col + hw, W - 1);
+            if (blo >= lb)
+                scan_staged<WORDS, NODUPES>(r, ldesc, lb, llen, lrow, blo, bhi, r1, r2);
+            else
+                scan_staged<WORDS, NODUPES>(r, ldesc, blo, 0, lrow, blo, bhi, r1, r2);
+            const int r0 = (int)(r1 & 0xFFFFu);
+            if (!(NODUPES && (r1 >> 16) == (r2 >> 16)) && abs(col - r0) <= max_lr_diff)
+                d = (col + r0) / 2 - best;
+        }
+    }
+
+    const FastCols c(col, d, W);
+    FastMoments m;
+#pragma unroll 8
+    for (int t = 0; t < n; ++t)
+        m.add((double)load_sample_cached<TIn>(f0, t, row, col), (double)load_sample_cached<TIn>(f1, t, row, c.cl),
+              (double)load_sample_cached<TIn>(f1, t, row, c.cmid), (double)load_sample_cached<TIn>(f1, t, row, c.cr));
+    bool lowvar;
+    const float res = subpixel_fast_decide(m, n, d, c.valid_d, c.interior, thr, npts, gx, has_mv != 0,
+                                           minvar, lowvar);
+    if (act) out[(size_t)row * ostep + col] = res;
+    if (!fallback) return;  // uniform
+    if (act) dwin[(size_t)row * W + col] = (int16_t)d;  // the windowed match, for set_listed
+    const bool take = act && __builtin_isnan(res) && !(lowvar && thr > -1.0f) && !(unsure && hw == hw_lo);
+    const uint64_t mk = __ballot(take);
+    if (lane == 0) wsum[wave] = __popcll(mk);
+    __syncthreads();
+    if (tid == 0) {
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        base_s = total ? atomicAdd(&row_count[row], total) : 0;
+    }
+    __syncthreads();
+    if (take) {
+        int off = base_s + __popcll(mk & ((1ull << lane) - 1));
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        list[(size_t)row * W + off] = (int16_t)col;
+    }
 }
 
 // Whether pixel (row, col) lies in a cell the network was unsure about: it
@@ -252,7 +443,7 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_search_list(
     if constexpr (MODE == SEARCH_REV) {
         res[(size_t)row * W + i] = valid ? (int16_t)best : INVALID16;
     } else {
-        if (col >= 0) disp[(size_t)row * dstep + col] = valid ? (int16_t)(col - best) : INVALID16;
+        if (col >= 0) set_listed(disp + (size_t)row * dstep + col, valid ? (int16_t)(col - best) : INVALID16);
     }
 }
 
@@ -270,7 +461,7 @@ __global__ __launch_bounds__(256) void k_lr_combine(const int16_t* __restrict__ 
         int16_t v = INVALID16;
         if (best != INVALID16 && r0 != INVALID16 && abs(col - r0) <= max_lr_diff)
             v = (int16_t)((col + r0) / 2 - best);
-        disp[(size_t)row * dstep + col] = v;
+        set_listed(disp + (size_t)row * dstep + col, v);
     }
 }
 

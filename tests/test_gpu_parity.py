@@ -789,6 +789,37 @@ def test_precision_moments_decisions_exact(gpu, oracle, which):
     assert same > (0.99 if which == "1920x1200_u16" else 0.9)
 
 
+@pytest.mark.parametrize("which", ["1920x1200_u16", "640x480_u8"])
+def test_precision_moments_fast_vs_exact(gpu, oracle, which):
+    """MOMENTS_FAST (closed form decided on its own values) against the exact
+    FP32 path on a 1920-wide crop: the share of pixels whose accept/reject
+    decision flips is at most 0.1 % (u16) / 2 % (u8, one grid step of 0.25
+    px), no valid pixel moves by more than one grid step for 99.5 % of the
+    pixels valid in both, and the result does not depend on the corrmap
+    request being absent (with a corrmap the exact stage runs)."""
+    from libbicos_amd import scene
+    if which == "1920x1200_u16":
+        L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=116, height=96)
+        kw = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+        flip_max, step = 1e-3, 0.1
+    else:
+        L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=117, height=64)
+        kw = dict(nxcorr_threshold=0.9, subpixel_step=0.25, mode=gpu.TransformMode.Full)
+        flip_max, step = 2e-2, 0.25
+    got = host(gpu.match_device(dev(L), dev(R), gpu.Config(precision=gpu.Precision.MomentsFast, **kw))[0])
+    ex, _ = run_match(gpu, L, R, gpu.Config(**kw))
+    flips = (np.isfinite(got) != np.isfinite(ex)).mean()
+    both = np.isfinite(got) & np.isfinite(ex)
+    near = (np.abs(got[both] - ex[both]) <= step * 1.0001).mean()
+    print(f"MOMENTS_FAST {which}: decision flips {flips:.2e}, within one grid step {near:.5f}, "
+          f"exact grid point {(got[both] == ex[both]).mean():.4f}")
+    assert flips <= flip_max and near >= 0.995
+    # with a corrmap: the exact FP32 stage (equal to SINGLE bit for bit)
+    d_cm, cm = gpu.match_device(dev(L), dev(R), gpu.Config(precision=gpu.Precision.MomentsFast, **kw),
+                                want_corrmap=True)
+    assert same(host(d_cm), ex)
+
+
 def test_precision_half_mfma_u8_full_mode(gpu, oracle):
     from libbicos_amd import scene
     L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=111, height=64)

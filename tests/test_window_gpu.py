@@ -239,12 +239,14 @@ def c1_crop(W, seed, H=64, Wd=480, frames=33):
     return L, R
 
 
-@pytest.mark.parametrize("prec", ["single", "moments"])
+@pytest.mark.parametrize("prec", ["single", "moments", "moments_fast"])
 @pytest.mark.parametrize("variant", ["nodup", "cons"])
 @pytest.mark.parametrize("subpix", [True, False])
 def test_full_width_window_equals_plain(W, variant, subpix, prec):
     """A window spanning the row makes the windowed search the full search:
-    the learned path must then reproduce the plain path bit for bit."""
+    the learned path must then reproduce the plain path bit for bit (for
+    MOMENTS_FAST without a corrmap: the fused windowed pass, its windows
+    longer than the staged columns)."""
     import libbicos_amd as B
     L, R = c1_crop(W, 31)
     kw = dict(BENCH_CFG)
@@ -252,6 +254,8 @@ def test_full_width_window_equals_plain(W, variant, subpix, prec):
         kw.pop("subpixel_step")
     if prec == "moments":
         kw["precision"] = B.Precision.Moments
+    if prec == "moments_fast":
+        kw["precision"] = B.Precision.MomentsFast
     if variant == "cons":
         kw["variant"] = B.Variant.Consistency(max_lr_diff=1, no_dupes=False)
     cfg = B.Config(**kw)
@@ -269,7 +273,7 @@ def test_full_width_window_equals_plain(W, variant, subpix, prec):
     assert np.array_equal(a, b, equal_nan=True)
 
 
-@pytest.mark.parametrize("impl", ["mfma", "mfma_i8", "valu", "moments"])
+@pytest.mark.parametrize("impl", ["mfma", "mfma_i8", "valu", "moments", "moments_fast", "moments_fast_valu"])
 @pytest.mark.parametrize("variant", ["nodup", "cons"])
 @pytest.mark.parametrize("subpix", [True, False])
 @pytest.mark.parametrize("hw", [0, 3])
@@ -280,11 +284,14 @@ def test_fallback_invariant_untrained(W, variant, subpix, hw, impl):
     pass the NCC threshold at another disparity. Every row-search kernel
     (fp4 and int8 matrix cores, vector ALU) serves the listed pixels."""
     import libbicos_amd as B
-    if impl == "moments" and not subpix:
-        pytest.skip("the MOMENTS precision concerns the subpixel stage")
-    B.set_search_impl({"valu": B.SEARCH_VALU, "mfma_i8": B.SEARCH_MFMA_I8}.get(impl, B.SEARCH_MFMA))
+    if impl.startswith("moments") and not subpix:
+        pytest.skip("the MOMENTS precisions concern the subpixel stage")
+    B.set_search_impl({"valu": B.SEARCH_VALU, "mfma_i8": B.SEARCH_MFMA_I8,
+                       "moments_fast_valu": B.SEARCH_VALU}.get(impl, B.SEARCH_MFMA))
     try:
-        _fallback_invariant(W, B, variant, subpix, hw, impl == "moments")
+        _fallback_invariant(W, B, variant, subpix, hw,
+                            {"moments": B.Precision.Moments, "moments_fast": B.Precision.MomentsFast,
+                             "moments_fast_valu": B.Precision.MomentsFast}.get(impl))
     finally:
         B.set_search_impl(B.SEARCH_MFMA)
 
@@ -294,15 +301,15 @@ def _fallback_invariant(W, B, variant, subpix, hw, moments):
     kw = dict(BENCH_CFG)
     if not subpix:
         kw.pop("subpixel_step")
-    if moments:
-        kw["precision"] = B.Precision.Moments
+    if moments is not None:  # the plain path runs the same precision
+        kw["precision"] = moments
     if variant == "cons":
         kw["variant"] = B.Variant.Consistency(max_lr_diff=1, no_dupes=False)
     cfg = B.Config(**kw)
     params = W.params_device(W.init_params(2))
     opts = W.WindowOptions(hw_lo=hw, hw_hi=hw)
     d_plain, _ = B.match_device(L, R, cfg)
-    if moments:  # no corrmap: the MOMENTS stage proper
+    if moments is not None:  # no corrmap: the MOMENTS stages proper
         d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, stats=True)
     else:
         d_win, c_win, st = W.match_windowed_device(L, R, cfg, params, opts, want_corrmap=True, stats=True)
@@ -481,16 +488,16 @@ def test_trained_windows_contain_labels(W):
     assert hit >= 0.97 and hit0 <= 0.5
 
 
-@pytest.mark.parametrize("prec", ["moments", "half_mfma"])
+@pytest.mark.parametrize("prec", ["moments_fast", "moments", "half_mfma"])
 def test_trained_agreement_full_size_1920x1200x33(W, prec):
     """1920x1200x33 u16, unseen scenes: >= 99.5 % of the plain-valid pixels
-    are valid in the learned path and within 0.5 px. MOMENTS with fallback
-    ALL is the configuration the benchmark reports."""
+    are valid in the learned path and within 0.5 px. MOMENTS_FAST with
+    fallback ALL is the configuration the benchmark reports."""
     import libbicos_amd as B
     from libbicos_amd import scene
     params, _ = trained(W)
-    cfg = B.Config(precision=B.Precision.Moments if prec == "moments" else B.Precision.HalfMFMA,
-                   **BENCH_CFG)
+    cfg = B.Config(precision={"moments_fast": B.Precision.MomentsFast, "moments": B.Precision.Moments,
+                              "half_mfma": B.Precision.HalfMFMA}[prec], **BENCH_CFG)
     for k in range(2):
         L, R, _ = W.scene_device(scene.SCENE_1920x1200_U16, W.EVAL_SEED0 + 100 + k)
         a = host(B.match_device(L, R, B.Config(**BENCH_CFG))[0])
