@@ -30,6 +30,8 @@ METRIC = "matched Mpixel/s @1920x1200x33 u16; learned vs plain at >=99.5% px agr
 HBM_PEAK_GBS = 8000.0
 INT8_DENSE_TOPS = 5000.0
 FP4_DENSE_TFLOPS = 10100.0
+FP32_TFLOPS = 157.3
+SUBPIXEL_OPS_PER_SAMPLE = 13   # interpolate, round, cast, centre, two fused steps, ... per grid point and frame
 VALU_SIMDS = 1024        # 256 CUs x 4 SIMDs; a wave64 VALU instruction takes 4 cycles
 CLOCK_HZ = 2.4e9
 
@@ -53,40 +55,45 @@ def parse_args():
     return p.parse_args()
 
 
-def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int, fp4: bool = False):
+def kernel_roofline(name: str, avg_ms: float, W: int, H: int, n: int, words: int, fp4: bool = False,
+                    npts: int = 20):
     """Algorithmic work of one launch of `name` against the MI355X roof.
 
     search: W*W*H candidates x (descriptor bits) multiply-accumulates (2 ops
     each) against the dense matrix peak of the operand type the kernel used
-    (fp4: 10.1 PFLOP/s; int8: 5.0 POP/s); transform / agree: bytes the
-    algorithm must move (samples read, outputs written) against HBM
-    bandwidth."""
+    (fp4: 10.1 PFLOP/s; int8: 5.0 POP/s); subpixel stages: W*H pixels x
+    npts grid points x n frames x 13 operations (the reference's scan: the
+    parabola, rint, the cast, centring and the two fused steps of the NCC
+    chains, agree.hpp:143-171) against the FP32 peak (157.3 TFLOP/s);
+    transform / agree: bytes the algorithm must move (samples read, outputs
+    written) against HBM bandwidth. Where rocprofv3 SQ counters for the
+    kernel are committed (profiles/pmc_traffic.txt), `valu_issue` adds the
+    vector-ALU issue utilisation: instructions per wave x waves x 4 cycles
+    over 1024 SIMDs at 2.4 GHz, over the launch time."""
     s = avg_ms * 1e-3
+    extra = {}
+    valu = committed_valu(name)
+    if valu is not None:
+        per_wave, waves = valu
+        extra["valu_issue"] = round(per_wave * waves * 4 / (VALU_SIMDS * CLOCK_HZ) / s, 4)
     if name.startswith("search"):
         ops = 2.0 * W * W * H * 32 * words
         peak = FP4_DENSE_TFLOPS if fp4 else INT8_DENSE_TOPS
         return {"bound": "mfma", "achieved": ops / s / 1e12, "peak": peak,
                 "unit": "TFLOP/s", "frac": ops / s / 1e12 / peak,
-                "work_per_launch": ops, "work_unit": "fp4 MACs x 2" if fp4 else "int8 MACs x 2"}
+                "work_per_launch": ops, "work_unit": "fp4 MACs x 2" if fp4 else "int8 MACs x 2", **extra}
     if name.startswith("agree_subpixel"):
-        # bound by vector-ALU issue: instructions per wave (rocprofv3
-        # SQ_INSTS_VALU / SQ_WAVES, committed) x waves x 4 cycles against
-        # 1024 SIMDs at 2.4 GHz
-        valu = committed_valu(name)
-        if valu is not None:
-            per_wave, waves = valu
-            issue_s = per_wave * waves * 4 / (VALU_SIMDS * CLOCK_HZ)
-            return {"bound": "valu", "achieved": issue_s / s, "peak": 1.0, "unit": "issue fraction",
-                    "frac": issue_s / s, "work_per_launch": per_wave * waves,
-                    "work_unit": "wave64 VALU instructions"}
+        ops = float(W) * H * npts * n * SUBPIXEL_OPS_PER_SAMPLE
+        return {"bound": "valu", "achieved": ops / s / 1e12, "peak": FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": ops / s / 1e12 / FP32_TFLOPS, "work_per_launch": ops,
+                "work_unit": f"pixels x {npts} grid points x {n} frames x {SUBPIXEL_OPS_PER_SAMPLE} FP32 ops",
+                **extra}
     if name.startswith("transform"):
         b = 2 * W * H * (n * 2 + 4 * words)  # one launch transforms both views
-    elif name.startswith("agree_subpixel"):
-        b = W * H * (2 + 4 * n * 2 + 4)      # disparity in, 4n samples, float out
     else:
         b = W * H * (2 + 2 * n * 2 + 2)
     return {"bound": "hbm", "achieved": b / s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": b / s / 1e9 / HBM_PEAK_GBS, "work_per_launch": b, "work_unit": "bytes"}
+            "frac": b / s / 1e9 / HBM_PEAK_GBS, "work_per_launch": b, "work_unit": "bytes", **extra}
 
 
 def committed_traffic(kernel: str):
@@ -140,11 +147,32 @@ def cpu_baseline(L, R, cfg_kw):
                 break
     except OSError:
         pass
+    # the reference's CPU configuration: 640x480 u8, 12 frames, FULL
+    # descriptors, threshold 0.9, no subpixel, no min_variance, NoDuplicates
+    from libbicos_amd import scene
+    L0, R0, _ = scene.generate(scene.SCENE_640x480_U8, seed=0x600DF00D)
+    c0_kw = dict(thr=0.9, step=None, min_variance=None, full=True)
+    O.match(L0, R0, **c0_kw)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.match(L0, R0, **c0_kw)
+    dc0 = (time.perf_counter() - t0) / reps
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.match(L0, R0, **c0_kw)
+    dc01 = time.perf_counter() - t0
+    O.set_threads(threads)
     return {"value": W * H / dt / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
             "sample": f"the full {W}x{H}x{n} workload once ({dt:.2f} s wall, "
                       f"{threads} threads, oracle/bicos_oracle.cpp)",
             "single_thread": {"value": W * band / dt1 / 1e6, "unit": "Mpixel/s",
                               "sample": f"rows 0..{band - 1} of the same pair ({dt1:.2f} s wall)"},
+            "c0_640x480x12_u8_full": {
+                "value": 640 * 480 / dc0 / 1e6, "unit": "Mpixel/s", "cores": threads,
+                "ms": round(dc0 * 1e3, 2), "single_thread_ms": round(dc01 * 1e3, 2),
+                "sample": f"BICOS::match on the 640x480 u8 12-frame pair (FULL, threshold 0.9, "
+                          f"no subpixel), mean of {reps} runs on {threads} threads; one run on 1 thread"},
             "cpu_model": model}
 
 
@@ -177,23 +205,28 @@ def pointcloud_timing(B, disp, args):
 
 
 def learned_path(B, dL, dR, plain_disp, args, W, H, plain_ms):
-    """The learned-window path (MOMENTS subpixel stage, committed weights)
-    on the same resident pair: timed like the plain path, then compared with
-    the plain disparity (share of plain-valid pixels valid and within 0.5 px)."""
+    """The learned-window path (MOMENTS_FAST subpixel stage, committed
+    weights) on the same resident pair: timed like the plain path, then
+    compared with the plain disparity (share of plain-valid pixels valid and
+    within 0.5 px); fallback ALL is the mode the target is stated for. ALL
+    with the MOMENTS stage (reference accept/reject decisions) is timed
+    beside it."""
     import torch
     from libbicos_amd import window as Wn
     if not os.path.exists(Wn.DEFAULT_WEIGHTS):
         return None
     params = Wn.params_device(Wn.load()[0])
-    cfg = B.Config(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0,
-                   precision=B.Precision.Moments)
+    base = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    fast = B.Config(precision=B.Precision.MomentsFast, **base)
+    mom = B.Config(precision=B.Precision.Moments, **base)
     stream = torch.cuda.current_stream()
     a = plain_disp
     va = torch.isfinite(a)
-    res = {"weights": os.path.relpath(Wn.DEFAULT_WEIGHTS, ROOT), "precision": "MOMENTS",
+    res = {"weights": os.path.relpath(Wn.DEFAULT_WEIGHTS, ROOT), "precision": "MOMENTS_FAST",
            "agreement_rule": "plain-valid pixels valid and within 0.5 px of plain"}
-    for name, fb in (("fallback_all", Wn.FALLBACK_ALL), ("fallback_unsure", Wn.FALLBACK_UNSURE),
-                     ("fallback_none", Wn.FALLBACK_NONE)):
+    for name, fb, cfg in (("fallback_all", Wn.FALLBACK_ALL, fast), ("fallback_unsure", Wn.FALLBACK_UNSURE, fast),
+                          ("fallback_none", Wn.FALLBACK_NONE, fast),
+                          ("fallback_all_moments", Wn.FALLBACK_ALL, mom)):
         opts = Wn.WindowOptions(fallback=fb)
         out, _, st = Wn.match_windowed_device(dL, dR, cfg, params, opts, stats=True, stream=stream)
         for _ in range(args.warmup):
@@ -223,6 +256,53 @@ def learned_path(B, dL, dR, plain_disp, args, W, H, plain_ms):
             "kernels_ms": {k: round(t / args.steps, 4) for k, (c, t) in sorted(kst.items(), key=lambda kv: -kv[1][1])},
         }
     return res
+
+
+def precision_modes(B, L, R, dL, dR, args, W, H):
+    """c2: the subpixel precision modes on the benchmark pair against the FP64
+    oracle (Precision::DOUBLE semantics, agree.cuh:35-65; the oracle on the
+    host's cores): max / mean |dNCC| (correlation map; HALF_MFMA rescored
+    winners, the exact stage's for the others), max / mean |d disparity|
+    over pixels valid in both, the share of pixels whose accept/reject
+    decision differs, and ms per match (inputs resident, no corrmap)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    base = dict(nxcorr_threshold=0.96, subpixel_step=0.1, min_variance=2.0)
+    exp, exp_cm = O.match(L, R, thr=0.96, step=0.1, min_variance=2.0, double=True, corrmap=True)
+    fe = np.isfinite(exp)
+    out = {"reference": "FP64 oracle on the same pair"}
+    for name, prec in (("SINGLE", B.Precision.Single), ("HALF_MFMA", B.Precision.HalfMFMA),
+                       ("MOMENTS", B.Precision.Moments), ("MOMENTS_FAST", B.Precision.MomentsFast)):
+        cfg = B.Config(precision=prec, **base)
+        d, _ = B.match_device(dL, dR, cfg)
+        for _ in range(5):
+            d, _ = B.match_device(dL, dR, cfg, disparity=d)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            d, _ = B.match_device(dL, dR, cfg, disparity=d)
+        e1.record()
+        torch.cuda.synchronize()
+        got = d.cpu().numpy()
+        fg = np.isfinite(got)
+        both = fg & fe
+        dd = np.abs(got[both].astype(np.float64) - exp[both])
+        rec = {"ms": round(e0.elapsed_time(e1) / args.steps, 4),
+               "d_max": float(dd.max()), "d_mean": float(dd.mean()),
+               "flip_frac": float((fg != fe).mean())}
+        if name in ("SINGLE", "HALF_MFMA"):
+            # the correlation map of this precision (HALF_MFMA: its rescored winners)
+            _, cm = B.match_device(dL, dR, cfg, want_corrmap=True)
+            cm = cm.cpu().numpy()
+            ok = np.isfinite(cm) & np.isfinite(exp_cm)
+            dn = np.abs(cm[ok].astype(np.float64) - exp_cm[ok])
+            rec["ncc_max"] = float(dn.max())
+            rec["ncc_mean"] = float(dn.mean())
+        out[name] = rec
+    return out
 
 
 # The reference's published kernel times on an RTX 4090 at 3300x2200 random
@@ -358,8 +438,8 @@ def stress(args):
                        "max_lr_diff 1, subpixel_step 0.05, nxcorr_threshold 0.96, min_variance 2.0",
            "stacks_bytes": free0 - free1}
     runs = [("plain", None, B.Config(**kw)),
-            ("learned_fallback_all", True, B.Config(precision=B.Precision.Moments, **kw)),
-            ("learned_fallback_none", False, B.Config(precision=B.Precision.Moments, **kw))]
+            ("learned_fallback_all", True, B.Config(precision=B.Precision.MomentsFast, **kw)),
+            ("learned_fallback_none", False, B.Config(precision=B.Precision.MomentsFast, **kw))]
     params = Wn.params_device(Wn.load()[0]) if os.path.exists(Wn.DEFAULT_WEIGHTS) else None
     plain = None
     for name, fb, cfg in runs:
@@ -468,11 +548,14 @@ def main():
         r = kernel_roofline(k, v["avg_ms"], W, H, n, words, fp4)
         v["bound"] = r["bound"]
         v["roofline_frac"] = round(r["frac"], 4)
+        if "valu_issue" in r:
+            v["valu_issue"] = r["valu_issue"]
 
     d = disp.cpu().numpy()
     valid = float(np.isfinite(d).mean())
     learned = learned_path(B, dL, dR, disp, args, W, H, ms_per_step)
     points = pointcloud_timing(B, disp, args)
+    precision = None if args.no_cpu_baseline else precision_modes(B, L, R, dL, dR, args, W, H)
 
     out = {
         "metric": METRIC,
@@ -497,6 +580,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "learned": learned,
+        "precision_c2": precision,
         "pointcloud": points,
     }
     if not args.no_cpu_baseline:

@@ -69,16 +69,55 @@ __global__ __launch_bounds__(256) void k_cnn_input(const Stack f, int W, int Hq,
     for (int t = 0; t < CNN_IN; ++t) x0[t * plane + (size_t)Y * Wq + X] = bf16_round((q[t] - m) * in_scale);
 }
 
+// Inference weights as BF16 bits: layer 0 in [co][k] with k = tap * 4 + ci
+// (36 values, zero-padded to CNN_K0 = 48: three MFMA steps of 16), layers
+// 1..3 in [layer-1][co][tap][ci] so that one 16-byte load gives 8
+// consecutive input channels of one tap.
+constexpr int CNN_K0 = 48;
+constexpr int CNN_MFMA_W0 = CNN_C * CNN_K0;
+constexpr int CNN_MFMA_W = 3 * CNN_C * 9 * CNN_C;
+
+__device__ __forceinline__ void cnn_pack_one(int i, const float* __restrict__ p, uint16_t* __restrict__ w0,
+                                             uint16_t* __restrict__ wm) {
+    if (i < CNN_MFMA_W0) {
+        const int co = i / CNN_K0, k = i % CNN_K0, tap = k / CNN_IN, ci = k % CNN_IN;
+        w0[i] = tap < 9 ? bf16_bits(p[cnn_w_off(0) + (co * CNN_IN + ci) * 9 + tap]) : (uint16_t)0;
+    }
+    if (i < CNN_MFMA_W) {
+        const int l = i / (CNN_C * 9 * CNN_C), r = i % (CNN_C * 9 * CNN_C);
+        const int co = r / (9 * CNN_C), tap = (r / CNN_C) % 9, ci = r % CNN_C;
+        wm[i] = bf16_bits(p[cnn_w_off(l + 1) + (co * CNN_C + ci) * 9 + tap]);
+    }
+}
+
+__global__ void k_cnn_pack_mfma(const float* __restrict__ p, uint16_t* __restrict__ w0,
+                                uint16_t* __restrict__ wm) {
+    cnn_pack_one(blockIdx.x * 256 + threadIdx.x, p, w0, wm);
+}
+
 // k_cnn_input for both views at once (view = blockIdx.z), written
 // cell-major as BF16 bits [Y][X][CNN_IN]; VEC: the 4 samples of a cell row
 // come in one load (stack base, frame stride and row step all multiples of
-// 4 samples). Same arithmetic as k_cnn_input.
+// 4 samples). Same arithmetic as k_cnn_input. The blocks of z = 2 pack the
+// inference weights (k_cnn_pack_mfma's work) and clear zero[0 .. nzero)
+// instead (a caller's counters, e.g. the fallback lists').
 template<typename TIn, bool VEC>
 __global__ __launch_bounds__(256) void k_cnn_input_pair(const Stack fa, const Stack fb, int Hq,
                                                         int Wq, float in_scale,
                                                         uint16_t* __restrict__ x0a,
-                                                        uint16_t* __restrict__ x0b) {
+                                                        uint16_t* __restrict__ x0b,
+                                                        const float* __restrict__ params,
+                                                        uint16_t* __restrict__ w0,
+                                                        uint16_t* __restrict__ wm,
+                                                        uint32_t* __restrict__ zero, int nzero) {
     static_assert(CNN_DS == 4, "one 4-sample load per cell row");
+    if (blockIdx.z == 2) {
+        const int threads = gridDim.x * gridDim.y * 256;
+        const int first = (blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
+        for (int i = first; i < CNN_MFMA_W; i += threads) cnn_pack_one(i, params, w0, wm);
+        for (int i = first; i < nzero; i += threads) zero[i] = 0u;
+        return;
+    }
     const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
     if (X >= Wq)
         return;
@@ -403,28 +442,6 @@ __global__ __launch_bounds__(256) void k_cnn_window(const float* __restrict__ C,
 }
 
 /* ------------------------------------------------------------ inference */
-
-// Inference weights as BF16 bits: layer 0 in [co][k] with k = tap * 4 + ci
-// (36 values, zero-padded to CNN_K0 = 48: three MFMA steps of 16), layers
-// 1..3 in [layer-1][co][tap][ci] so that one 16-byte load gives 8
-// consecutive input channels of one tap.
-constexpr int CNN_K0 = 48;
-constexpr int CNN_MFMA_W0 = CNN_C * CNN_K0;
-constexpr int CNN_MFMA_W = 3 * CNN_C * 9 * CNN_C;
-
-__global__ void k_cnn_pack_mfma(const float* __restrict__ p, uint16_t* __restrict__ w0,
-                                uint16_t* __restrict__ wm) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < CNN_MFMA_W0) {
-        const int co = i / CNN_K0, k = i % CNN_K0, tap = k / CNN_IN, ci = k % CNN_IN;
-        w0[i] = tap < 9 ? bf16_bits(p[cnn_w_off(0) + (co * CNN_IN + ci) * 9 + tap]) : (uint16_t)0;
-    }
-    if (i < CNN_MFMA_W) {
-        const int l = i / (CNN_C * 9 * CNN_C), r = i % (CNN_C * 9 * CNN_C);
-        const int co = r / (9 * CNN_C), tap = (r / CNN_C) % 9, ci = r % CNN_C;
-        wm[i] = bf16_bits(p[cnn_w_off(l + 1) + (co * CNN_C + ci) * 9 + tap]);
-    }
-}
 
 // layer 0 (4 -> 32 channels, ReLU) on v_mfma_f32_32x32x16_bf16, from the
 // cell-major x0 of view blockIdx.z: one wave per 32-cell tile of a row,

@@ -428,10 +428,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
     const int row = LIST ? blockIdx.x : blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nl = lane & 31, g = lane >> 5;
-    const int group = LIST ? blockIdx.y : blockIdx.x;
-    int count = 0;
+    // LIST: a block serves groups blockIdx.y, blockIdx.y + gridDim.y, ...
+    // of WAVES * NB * 32 entries of its row (staging the row again per group)
+    const int count = LIST ? row_count[row] : 0;
+    for (int group = LIST ? blockIdx.y : blockIdx.x;; group += gridDim.y) {
     if constexpr (LIST) {
-        count = row_count[row];
         if (group * WAVES * NB * 32 >= count)
             return;  // uniform over the block
     }
@@ -592,6 +593,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
             }
         }
         out[(size_t)row * ostep + col] = v;
+    }
+    if constexpr (!LIST) return;
+    __syncthreads();  // the next group stages the row again
     }
 }
 

@@ -159,9 +159,10 @@ size_t windows_scratch_bytes(const Grid& g) {
 
 // Windows of all cells into center/halfwidth (peak nullable); scratch holds
 // windows_scratch_bytes(g).
+// zero (nullable): nzero counters the network's first kernel clears on the way
 int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, float in_scale,
                 const float* params, const bicos_window_options* o, void* scratch, float* center,
-                int16_t* halfwidth, float* peak, hipStream_t s) {
+                int16_t* halfwidth, float* peak, hipStream_t s, int* zero = nullptr, int nzero = 0) {
     const dim3 grid((g.Wq + 255) / 256, g.Hq), block(256);
     if (g_cnn_impl == BICOS_CNN_MFMA || g_cnn_impl == BICOS_CNN_MFMA_LAYERS) {
         if (g.K > CNN_KMAX_MFMA)
@@ -175,15 +176,15 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
         uint16_t* bL = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
         uint16_t* aR = (uint16_t*)p; p += align_up(CNN_C * g.cells * 2, 256);
         uint16_t* bR = (uint16_t*)p;
-        LAUNCH("cnn_weights", k_cnn_pack_mfma, dim3((CNN_MFMA_W + 255) / 256), block, 0, s, params, w0, wm);
         // one 4-sample load per cell row where both stacks allow it
         const size_t vb = 4 * (depth == BICOS_8U ? 1 : 2);
         const auto aligned = [&](const Stack& v) {
             return ((uintptr_t)v.base % vb) == 0 && ((size_t)v.fstride % vb) == 0 && ((size_t)v.step % vb) == 0;
         };
         const bool vec = aligned(v0) && aligned(v1);
-        const dim3 igrid((g.Wq + 255) / 256, g.Hq, 2);
-#define CI(T, V) LAUNCH("cnn_input", (k_cnn_input_pair<T, V>), igrid, block, 0, s, v0, v1, g.Hq, g.Wq, in_scale, x0L, x0R)
+        // both views' inputs, and the weights packed by the blocks of z = 2
+        const dim3 igrid((g.Wq + 255) / 256, g.Hq, 3);
+#define CI(T, V) LAUNCH("cnn_input", (k_cnn_input_pair<T, V>), igrid, block, 0, s, v0, v1, g.Hq, g.Wq, in_scale, x0L, x0R, params, w0, wm, (uint32_t*)zero, zero ? nzero : 0)
         if (depth == BICOS_8U) {
             if (vec) CI(uint8_t, true); else CI(uint8_t, false);
         } else {
@@ -324,8 +325,14 @@ int run_search_list_mfma4_w(const uint32_t* dq, const uint32_t* dc, int H, int W
                             int mode, const int16_t* qcol, const int* row_count, int16_t* res,
                             int16_t* disp, int64_t dstep, hipStream_t s) {
     constexpr size_t lds = (size_t)CH * (WORDS + 1) * 16;
-    const dim3 grid(H, (W + WAVES * NB * 32 - 1) / (WAVES * NB * 32)), block(WAVES * 64);
-#define SM(ND, M) LAUNCH("search_list_mfma", (k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH, true>), grid, block, lds, s, dq, dc, W, (const int16_t*)nullptr, 0, disp, dstep, qcol, row_count, res)
+    // one block per row, looping over the row's groups of entries (a row
+    // lists a tenth of its pixels or so: mostly a single group)
+    const dim3 grid(H, 1), block(WAVES * 64);
+#define SM(ND, M) do { \
+        int rc_ = allow_dynamic_lds((const void*)k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH, true>, lds); \
+        if (rc_) return rc_; \
+        LAUNCH("search_list_mfma", (k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH, true>), grid, block, lds, s, dq, dc, W, (const int16_t*)nullptr, 0, disp, dstep, qcol, row_count, res); \
+    } while (0)
     if (nodupes) {
         if (mode == SEARCH_REV) SM(true, SEARCH_REV); else SM(true, SEARCH_DISP);
     } else {
@@ -344,7 +351,7 @@ int run_search_list(const uint32_t* dq, const uint32_t* dc, int H, int W, int wo
     if (g_search_impl == BICOS_SEARCH_MFMA && words >= 2 && W <= EMBED_MAX_W) {
         switch (words) {
             case 2: return run_search_list_mfma4_w<2, 1, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
-            case 4: return run_search_list_mfma4_w<4, 1, 8, 800>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 4: return run_search_list_mfma4_w<4, 1, 8, 960>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
             case 8: return run_search_list_mfma4_w<8, 1, 8, 448>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
@@ -715,13 +722,17 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         if ((r = side_stream(dev, &ss))) return r;
         HIP_TRY(hipEventRecord(ss->fork, s));
         HIP_TRY(hipStreamWaitEvent(ss->stream, ss->fork, 0));
+        // the fused pass appends to the row lists, which the network's first
+        // kernel clears (the vector-ALU network: one more kernel)
+        const bool clear = fast && fallback;
         int rc_zero = BICOS_OK;
-        if (fast && fallback)  // the fused pass appends to the row lists
+        if (clear && g_cnn_impl == BICOS_CNN_VALU)
             rc_zero = launch("fallback_list", k_zero_u32, dim3((H + 255) / 256), dim3(256), 0, ss->stream,
                              (unsigned int*)row_count, H);
         const int rc_cnn = rc_zero ? rc_zero
                                    : cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch,
-                                                 center, hw, nullptr, ss->stream);
+                                                 center, hw, nullptr, ss->stream,
+                                                 clear && g_cnn_impl != BICOS_CNN_VALU ? row_count : nullptr, H);
         // joined in every case: the workspace is released on the caller's stream
         HIP_TRY(hipEventRecord(ss->join, ss->stream));
         r = rc_cnn ? rc_cnn
