@@ -1012,6 +1012,59 @@ int oracle_subpixel_curve(int depth, int n, int64_t npix, const void* left, cons
     return 0;
 }
 
+/* The closed-form subpixel correlation the MOMENTS_FAST precision scores
+ * (no reference counterpart; the stage it stands in for is agree.hpp:94-185):
+ * for pixel i with left samples left[i * n + t] and right columns
+ * right3[(i * 3 + j) * n + t] (col1 - 1 + j), the correlation of the left
+ * samples with the unrounded parabola samples a x^2 + b x + c (the
+ * coefficients of agree.hpp:152-156) at the grid points of agree.hpp:162,
+ * in double precision straight from its definition: the interpolated
+ * samples are formed and centred, and the three sums taken. out[i * npts + k]
+ * (nxc), var1_out[i * npts + k] (the right variance, for the variance test). */
+int oracle_subpixel_closed_form(int depth, int n, int64_t npix, const void* left, const void* right3,
+                                float step, int npts, double* out, double* var1_out) {
+    if (!(step > 0.f) || n < 2 || n > 128) { g_err = "bad arguments"; return -1; }
+    std::vector<float> grid;
+    for (float x = -1.f; x <= 1.f && (int)grid.size() < npts; x += step) grid.push_back(x);
+    auto run = [&](auto tag) {
+        using TInput = decltype(tag);
+        const TInput* L = (const TInput*)left;
+        const TInput* R = (const TInput*)right3;
+        for (int64_t i = 0; i < npix; ++i) {
+            const TInput* p0 = L + i * n;
+            const TInput *y0 = R + (i * 3) * n, *y1 = y0 + n, *y2 = y1 + n;
+            double m0 = 0;
+            for (int t = 0; t < n; ++t) m0 += p0[t];
+            m0 /= n;
+            for (int k = 0; k < (int)grid.size(); ++k) {
+                const double x = grid[k];
+                std::vector<double> v(n);
+                double m1 = 0;
+                for (int t = 0; t < n; ++t) {
+                    const double a = 0.5 * ((double)y0[t] - 2.0 * y1[t] + y2[t]);
+                    const double b = 0.5 * ((double)y2[t] - (double)y0[t]);
+                    v[t] = a * x * x + b * x + (double)y1[t];
+                    m1 += v[t];
+                }
+                m1 /= n;
+                double cov = 0, var0 = 0, var1 = 0;
+                for (int t = 0; t < n; ++t) {
+                    const double d0 = p0[t] - m0, d1 = v[t] - m1;
+                    cov += d0 * d1;
+                    var0 += d0 * d0;
+                    var1 += d1 * d1;
+                }
+                out[i * npts + k] = cov / std::sqrt(var0 * var1);
+                var1_out[i * npts + k] = var1;
+            }
+        }
+    };
+    if (depth == 1) run(uint8_t{});
+    else if (depth == 2) run(uint16_t{});
+    else { g_err = "bad depth"; return -1; }
+    return 0;
+}
+
 /* Window-restricted row search (bicos_window above) on descriptor words.
  * center / halfwidth: (rows / 4) x (cols / 4) cell arrays. */
 int oracle_search_window(const uint32_t* d0, const uint32_t* d1, int rows, int cols, int words,

@@ -68,6 +68,7 @@ def lib():
                                             C.POINTER(C.c_int64)]
         L.oracle_interp_sample.argtypes = [i32, i32, i32, i32, f32]
         L.oracle_subpixel_curve.argtypes = [i32, i32, i64, vp, vp, f32, i32, vp]
+        L.oracle_subpixel_closed_form.argtypes = [i32, i32, i64, vp, vp, f32, i32, vp, vp]
         L.oracle_required_bits.argtypes = [i32, i32]
         L.oracle_descriptor_words.argtypes = [i32, i32]
         L.oracle_set_threads.argtypes = [i32]
@@ -203,6 +204,22 @@ def subpixel_curve(left: np.ndarray, right3: np.ndarray, step: float) -> np.ndar
     _check(lib().oracle_subpixel_curve(left.dtype.itemsize, n, P, left.ctypes.data,
                                        right3.ctypes.data, float(step), npts, out.ctypes.data))
     return out
+
+
+def subpixel_closed_form(left: np.ndarray, right3: np.ndarray, step: float):
+    """The closed-form correlation of MOMENTS_FAST (unrounded parabola
+    samples, double precision) at every grid point for a batch of pixels:
+    left (P, n), right3 (P, 3, n) -> (nxc (P, npts), var1 (P, npts))."""
+    left = np.ascontiguousarray(left)
+    right3 = np.ascontiguousarray(right3, left.dtype)
+    P, n = left.shape
+    npts = len(subpixel_grid(step))
+    out = np.empty((P, npts), np.float64)
+    var1 = np.empty((P, npts), np.float64)
+    _check(lib().oracle_subpixel_closed_form(left.dtype.itemsize, n, P, left.ctypes.data,
+                                             right3.ctypes.data, float(step), npts, out.ctypes.data,
+                                             var1.ctypes.data))
+    return out, var1
 
 
 def match(s0: np.ndarray, s1: np.ndarray, *, thr: float | None = 0.5, step: float | None = None,

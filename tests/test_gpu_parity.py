@@ -820,6 +820,83 @@ def test_precision_moments_fast_vs_exact(gpu, oracle, which):
     assert same(host(d_cm), ex)
 
 
+def closed_form_decisions(oracle, L, R, raw, thr, step, minvar_n, margin):
+    """What MOMENTS_FAST computes, in double precision from the oracle's
+    closed form: (expected float disparity, robust mask). A pixel is robust
+    where its decision cannot turn on the kernel's FP32 evaluation of the
+    closed form: best score farther than margin from the threshold and from
+    every other grid point's score, variances farther than 1 % from minvar."""
+    n, rows, cols = L.shape
+    grid = oracle.subpixel_grid(step)
+    exp = np.full((rows, cols), np.nan, np.float32)
+    robust = np.zeros((rows, cols), bool)
+    rr, cc = np.nonzero(raw != -32768)
+    c1 = cc - raw[rr, cc].astype(np.int64)
+    keep = (0 <= c1) & (c1 < cols)
+    rr, cc, c1 = rr[keep], cc[keep], c1[keep]
+    left = L[:, rr, cc].T
+    ys = np.stack([R[:, rr, np.clip(c1 + j, 0, cols - 1)].T for j in (-1, 0, 1)], axis=1)
+    nxc, var1 = oracle.subpixel_closed_form(left, ys, step)
+    lf = left.astype(np.float64)
+    var0 = ((lf - lf.mean(1, keepdims=True)) ** 2).sum(1)
+    border = (c1 == 0) | (c1 == cols - 1)
+    # border columns: the plain correlation at col1, which is the closed form at x = 0
+    k0 = int(np.argmin(np.abs(grid)))
+    scores = np.where(var1 < minvar_n, -1.0, nxc) if minvar_n is not None else nxc
+    scores = np.where(np.isnan(scores), -np.inf, scores)    # 0 / 0 never beats the best
+    low0 = (var0 < minvar_n) if minvar_n is not None else np.zeros(len(var0), bool)
+    best_k = np.argmax(scores, axis=1)                      # first maximum
+    best = scores[np.arange(len(best_k)), best_k]
+    srt = np.sort(scores, axis=1)
+    gap = np.where(best_k >= 0, srt[:, -1] - srt[:, -2], 1.0)
+    best = np.where(low0, -1.0, best)
+    x = np.where(low0 | border, np.float32(0), grid[best_k])
+    best = np.where(border & ~low0, scores[:, k0], best)
+    accept = best >= thr
+    d = raw[rr, cc].astype(np.float32)
+    exp[rr, cc] = np.where(accept, d - x, np.nan)
+    rob = np.abs(best - thr) > margin
+    rob &= border | low0 | (gap > margin)
+    if minvar_n is not None:
+        rob &= np.abs(var0 - minvar_n) > 0.01 * minvar_n
+        rob &= (np.abs(var1 - minvar_n) > 0.01 * minvar_n).all(1)
+    robust[rr, cc] = rob
+    # pixels without a valid match are NaN on every path
+    robust[raw == -32768] = True
+    return exp, robust
+
+
+@pytest.mark.parametrize("which", ["1920x1200_u16", "640x480_u8"])
+def test_moments_fast_follows_closed_form(gpu, oracle, which):
+    """The MOMENTS_FAST kernel (moments summed on the GPU, the parabola's
+    coefficients and the quartic variance evaluated in FP32) decides as the
+    closed form evaluated in double from its definition by the oracle, given
+    the oracle's integer search: on every pixel whose decision has a margin
+    above the FP32 evaluation error, bit for bit; a wrong coefficient of the
+    closed form moves the best grid point or the decision of most pixels."""
+    from libbicos_amd import scene
+    if which == "1920x1200_u16":
+        L, R, _ = scene.generate(scene.SCENE_1920x1200_U16, seed=118, height=48)
+        thr, step, mv, full = 0.96, 0.1, 2.0, False
+    else:
+        L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=119, height=48)
+        thr, step, mv, full = 0.9, 0.25, None, True
+    n = L.shape[0]
+    raw = oracle.search(oracle.transform(L, full), oracle.transform(R, full), 1)
+    # margin 1e-4: far above the FP32 evaluation error (coefficients rounded
+    # once to FP32, a degree-4 Horner scheme and rsq, a few 1e-7 relative)
+    exp, robust = closed_form_decisions(oracle, L, R, raw, thr, step, None if mv is None else mv * n, 1e-4)
+    cfg = gpu.Config(nxcorr_threshold=thr, subpixel_step=step, min_variance=mv,
+                     mode=gpu.TransformMode.Full if full else gpu.TransformMode.Limited,
+                     precision=gpu.Precision.MomentsFast)
+    got = host(gpu.match_device(dev(L), dev(R), cfg)[0])
+    valid = np.isfinite(exp)
+    print(f"MOMENTS_FAST {which}: {robust.mean():.4f} robust, {valid.mean():.3f} valid")
+    assert robust.mean() > 0.75 and (valid & robust).sum() > 0.75 * valid.sum()
+    bad = robust & ~(np.isnan(got) & np.isnan(exp) | (got == exp))
+    assert not bad.any(), (int(bad.sum()), np.argwhere(bad)[:5].tolist())
+
+
 def test_precision_half_mfma_u8_full_mode(gpu, oracle):
     from libbicos_amd import scene
     L, R, _ = scene.generate(scene.SCENE_640x480_U8, seed=111, height=64)

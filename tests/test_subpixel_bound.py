@@ -163,3 +163,26 @@ def test_bound_is_not_vacuous(oracle):
         assert (err > (d * d / sd) / sd + slack).any(), kind
         halves += int((err > 0.5 * (w * d + d * d / sd) / sd).sum())
     assert halves > 0
+
+
+@pytest.mark.parametrize("dtype,n", [(np.uint16, 33), (np.uint8, 12), (np.uint16, 64)])
+def test_closed_form_from_moments_equals_definition(oracle, dtype, n):
+    """The moment form above (sums, then the 3 x 3 Gram matrix) against the
+    oracle's closed form taken straight from its definition (interpolated
+    samples formed, centred, summed); at x = -1 the parabola passes through
+    the left column, so both equal the FP64 NCC of the two sample vectors."""
+    rng = np.random.default_rng(500 + n)
+    vmax = int(np.iinfo(dtype).max)
+    for kind in ("correlated", "near_threshold", "random"):
+        u, y = draw_pixels(rng, 4000, n, vmax, kind)
+        u, y = u.astype(dtype), y.astype(dtype)
+        grid = oracle.subpixel_grid(0.1)
+        r, var1, var0 = exact_closed_form(u, y, grid)
+        r_def, var1_def = oracle.subpixel_closed_form(u, y, 0.1)
+        ok = var0[:, None] * var1 > 0
+        assert np.allclose(r[ok], r_def[ok], rtol=0, atol=1e-9), kind
+        assert np.allclose(var1, var1_def, rtol=1e-9, atol=1e-6), kind
+        assert grid[0] == -1.0
+        for i in range(0, len(u), 97):
+            if ok[i, 0]:
+                assert abs(r_def[i, 0] - oracle.nxcorrd(u[i], y[i, 0])) < 1e-12
