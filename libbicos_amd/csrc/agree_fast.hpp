@@ -9,12 +9,13 @@
 //   cov(x)  = x^2 uA + x uB + uC                  (uA, uB, uC: centred cross moments)
 //   var1(x) = x^4 AA + 2 x^3 AB + x^2 (BB + 2 AC) + 2 x BC + CC   (from the 3 x 3 Gram matrix)
 // One lane per pixel sums the moments exactly (FP64 sums of integer
-// products), centres them in FP64 and scores every grid point from the closed
-// form in FP32. The scan's first strict maximum, the threshold and the variance
+// products), centres them exactly and scores every grid point from the
+// closed form. The scan's first strict maximum, the threshold and the variance
 // tests are taken on these values: there is no exact stage, so a pixel whose
-// best correlation lies within the closed form's error (about 1e-4 at
-// 33 frames) of the threshold can be decided the other way than by the
-// reference, and near-equal grid points can swap. The learned search window
+// best correlation lies within the closed form's distance from the
+// reference's rounded samples (about 1e-4 at 33 frames) of the threshold can
+// be decided the other way than by the reference, and near-equal grid points
+// can swap. The learned search window
 // measures what that costs against the plain path (agreement within 0.5 px).
 #pragma once
 
@@ -45,49 +46,65 @@ struct FastMoments {
 // column): res = d - x of the best grid point, or NaN; lowvar = the left
 // variance is below minvar (every match of the pixel is rejected when the
 // threshold is above -1).
+//
+// Every centred moment is formed as an integer scaled by n (n su0 - su s0,
+// ...), and the closed form's coefficients as integers scaled by 2n (cov) and
+// 4n (var1): all are exact in FP64 for the supported depths and sample
+// counts (below 2^53), so a right column of constant samples gives exact
+// zeros. The closed form is evaluated in FP64 at each grid point (FP64 FMA
+// issues at the FP32 rate here) and only the final ratio in FP32, which
+// keeps the score within a few 1e-7 of the closed form's value. A grid point
+// whose right variance is zero within that evaluation's rounding (below
+// 2^-30 of the three columns' total variance) is not scored, as the
+// reference's correlation of constant samples (0 / 0) never wins.
 __device__ __forceinline__ float subpixel_fast_decide(const FastMoments& m, int n, int d, bool valid_d,
                                                       bool interior, float thr, int npts, const GridX& gx,
                                                       bool mv, float minvar, bool& lowvar) {
-    const double inv = 1.0 / (double)n;
-    const double mu = m.su * inv;
-    const double var0 = m.suu - m.su * mu;
-    lowvar = mv && var0 < (double)minvar;
+    const double dn = (double)n;
+    const double v0 = dn * m.suu - m.su * m.su;           // n var0
+    lowvar = mv && v0 < dn * (double)minvar;
     if (!valid_d) return NAN;
 
-    const double m0 = m.s0 * inv, m1 = m.s1 * inv, m2 = m.s2 * inv;
-    const double u1 = m.su1 - m.s1 * mu;
-    const double g11 = m.s11 - m.s1 * m1;
-    const float rv0 = __builtin_amdgcn_rsqf((float)var0);
+    const double u1 = dn * m.su1 - m.su * m.s1;           // n (centred cross moment)
+    const double g11 = dn * m.s11 - m.s1 * m.s1;          // n (centred Gram entry)
+    const float rv0 = __builtin_amdgcn_rsqf((float)v0);
     if (!interior) {
         // border column: the plain correlation at col1 (agree.hpp:127-141)
-        const bool low1 = mv && g11 < (double)minvar;
+        const bool low1 = mv && g11 < dn * (double)minvar;
         const float corr = (lowvar || low1) ? -1.0f : (float)u1 * __builtin_amdgcn_rsqf((float)g11) * rv0;
         return !(corr < thr) ? (float)d : NAN;
     }
     if (lowvar)  // no grid point is scored: the best stays -1 at x = 0
         return !(-1.0f < thr) ? (float)d : NAN;
 
-    const double u0 = m.su0 - m.s0 * mu, u2 = m.su2 - m.s2 * mu;
-    const double g00 = m.s00 - m.s0 * m0, g01 = m.s01 - m.s0 * m1;
-    const double g02 = m.s02 - m.s0 * m2, g12 = m.s12 - m.s1 * m2;
-    const double g22 = m.s22 - m.s2 * m2;
+    const double u0 = dn * m.su0 - m.su * m.s0, u2 = dn * m.su2 - m.su * m.s2;
+    const double g00 = dn * m.s00 - m.s0 * m.s0, g01 = dn * m.s01 - m.s0 * m.s1;
+    const double g02 = dn * m.s02 - m.s0 * m.s2, g12 = dn * m.s12 - m.s1 * m.s2;
+    const double g22 = dn * m.s22 - m.s2 * m.s2;
     // cov(x) and var1(x) of the parabola a x^2 + b x + c through the three
-    // columns (a = (y0 - 2 y1 + y2) / 2, b = (y2 - y0) / 2, c = y1)
-    const float ua = (float)(0.5 * (u0 + u2) - u1), ub = (float)(0.5 * (u2 - u0)), uc = (float)u1;
-    const float aa = (float)(0.25 * (g00 + g22 + 2.0 * g02) + g11 - (g01 + g12));
-    const float ab2 = (float)(0.5 * (g22 - g00) + g01 - g12);
-    const float bbac = (float)(0.25 * (g00 + g22 - 2.0 * g02) + g01 + g12 - 2.0 * g11);
-    const float bc2 = (float)(g12 - g01);
-    const float cc = (float)g11;
+    // columns (a = (y0 - 2 y1 + y2) / 2, b = (y2 - y0) / 2, c = y1), as
+    // 2n cov = ua x^2 + ub x + uc and 4n var1 = aa x^4 + ab2 x^3 + bbac x^2 + bc2 x + cc
+    const double ua = u0 + u2 - 2.0 * u1, ub = u2 - u0, uc = 2.0 * u1;
+    const double aa = g00 + g22 + 2.0 * g02 + 4.0 * g11 - 4.0 * (g01 + g12);
+    const double ab2 = 2.0 * (g22 - g00) + 4.0 * (g01 - g12);
+    const double bbac = g00 + g22 - 2.0 * g02 + 4.0 * (g01 + g12) - 8.0 * g11;
+    const double bc2 = 4.0 * (g12 - g01);
+    const double cc = 4.0 * g11;
+    const double minvar4 = 4.0 * dn * (double)minvar;
+    const double zero_var = 0x1p-30 * 4.0 * (g00 + g11 + g22);
     float best = -1.0f, best_x = 0.0f;
     for (int k = 0; k < npts; ++k) {
-        const float x = gx.x[k];
-        const float cov = __builtin_fmaf(__builtin_fmaf(ua, x, ub), x, uc);
-        const float v1 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(__builtin_fmaf(aa, x, ab2), x, bbac), x, bc2), x, cc);
-        const float nxc = (mv && v1 < minvar) ? -1.0f : cov * __builtin_amdgcn_rsqf(v1) * rv0;
+        const float xf = gx.x[k];
+        const double x = (double)xf;
+        const double cov = __builtin_fma(__builtin_fma(ua, x, ub), x, uc);
+        const double v1 = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(aa, x, ab2), x, bbac), x, bc2), x, cc);
+        // 2n cov / sqrt(n var0 * 4n var1) = cov / sqrt(var0 var1)
+        float nxc = (float)cov * __builtin_amdgcn_rsqf((float)v1) * rv0;
+        if (!(v1 > zero_var)) nxc = NAN;
+        if (mv && v1 < minvar4) nxc = -1.0f;
         if (best < nxc) {  // first strict maximum (agree.hpp:168-171)
             best = nxc;
-            best_x = x;
+            best_x = xf;
         }
     }
     return !(best < thr) ? (float)d - best_x : NAN;

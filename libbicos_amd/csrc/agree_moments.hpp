@@ -325,28 +325,44 @@ __device__ void exact_candidates_wave(bool need, uint64_t cmask, int row, int co
         const float4* px = L.xs + p * NMAX;
         const TPrec mean0 = (TPrec)L.mean0[p];
         const bool fast = __ballot(take && !inrange) == 0;  // uniform over the wave
-        // frame t's interpolated sample at xe (agree.hpp:164)
-        const auto sample = [&](const float4& v) {
-            const float r = __builtin_rintf(v.y * xe * xe + v.z * xe + v.w);
-            return fast ? r : (float)(TIn)(int)r;
-        };
-        // both of the reference's passes recompute the samples (keeping them
-        // in registers would double the kernel's register count); the
-        // correlation in TPrec as nxcorr / nxcorrd (agree.cuh:35-97)
+        // frame t's interpolated sample at xe (agree.hpp:164), before the cast
+        const auto interp = [&](const float4& v) { return __builtin_rintf(v.y * xe * xe + v.z * xe + v.w); };
+        const auto wrap = [](float r) { return (float)(TIn)(int)r; };
+        // Both of the reference's passes read frame t's sample. The first
+        // forms it (with the cast only where a parabola of the round leaves
+        // the range; the choice is uniform over the wave) and, for stacks of
+        // at most 36 frames, keeps it for the second as a 16-bit value (the
+        // samples are integers in [0, max(TIn)]), two per register. The
+        // correlation in TPrec as nxcorr / nxcorrd (agree.cuh:35-97).
+        constexpr bool KEEP = NMAX <= 36;
+        constexpr int UNR = KEEP ? NMAX : 8;
+        uint32_t kept[KEEP ? (NMAX + 1) / 2 : 1] = {};
         TPrec mean1 = 0, covar = 0, var0 = 0, var1 = 0;
-#pragma unroll 8
-        for (int t = 0; t < NMAX; ++t)
-            if (n.has(t)) mean1 += (TPrec)sample(px[t]);
+        const auto pass1 = [&](auto cast) {
+#pragma unroll UNR
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) {
+                    const float smp = cast(interp(px[t]));
+                    mean1 += (TPrec)smp;
+                    if constexpr (KEEP) kept[t >> 1] |= (uint32_t)smp << (16 * (t & 1));
+                }
+        };
+        if (fast) pass1([](float r) { return r; });
+        else pass1(wrap);
         mean1 /= (TPrec)nf;
-#pragma unroll 8
-        for (int t = 0; t < NMAX; ++t)
-            if (n.has(t)) {
-                const float4 v = px[t];
-                const TPrec d0 = (TPrec)v.x - mean0, d1 = (TPrec)sample(v) - mean1;
-                var0 = pfma(d0, d0, var0);
-                covar = pfma(d0, d1, covar);
-                var1 = pfma(d1, d1, var1);
-            }
+        const auto pass2 = [&](auto sample) {
+#pragma unroll UNR
+            for (int t = 0; t < NMAX; ++t)
+                if (n.has(t)) {
+                    const TPrec d0 = (TPrec)px[t].x - mean0, d1 = (TPrec)sample(t) - mean1;
+                    var0 = pfma(d0, d0, var0);
+                    covar = pfma(d0, d1, covar);
+                    var1 = pfma(d1, d1, var1);
+                }
+        };
+        if constexpr (KEEP) pass2([&](int t) { return (float)((kept[t >> 1] >> (16 * (t & 1))) & 0xFFFFu); });
+        else if (fast) pass2([&](int t) { return interp(px[t]); });
+        else pass2([&](int t) { return wrap(interp(px[t])); });
         TPrec best = (mv && (var0 < (TPrec)minvar || var1 < (TPrec)minvar)) ? (TPrec)-1
                                                                              : covar / psqrt(var0 * var1);
         int best_k = k;
@@ -419,10 +435,11 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
     // branch-free loop (the right samples of lanes without an interior match
     // are read at a valid column and ignored), so that loads of many frames
     // are in flight at once
-    const int u_ref = (int)load_sample_cached<TIn>(f0, 0, row, col);
-    const int y_ref0 = (int)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
-    const int y_ref1 = (int)load_sample_cached<TIn>(f1, 0, row, c1);
-    const int y_ref2 = (int)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
+    // (BX: unshifted, the FP64 sums of integer products are exact)
+    const int u_ref = BX ? 0 : (int)load_sample_cached<TIn>(f0, 0, row, col);
+    const int y_ref0 = BX ? 0 : (int)load_sample_cached<TIn>(f1, 0, row, c1 - 1);
+    const int y_ref1 = BX ? 0 : (int)load_sample_cached<TIn>(f1, 0, row, c1);
+    const int y_ref2 = BX ? 0 : (int)load_sample_cached<TIn>(f1, 0, row, c1 + 1);
     // frames in flight per step; fewer for the exact FP64 sums (registers)
     constexpr int MOM_UNROLL = BX ? 8 : 11;
 #pragma unroll MOM_UNROLL
@@ -442,9 +459,9 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             const int r4 = (abs(i2 - i0) + 3) >> 2;
             ylo = min(ylo, min(min(i0, i2), i1 - r4));
             yhi = max(yhi, max(max(i0, i2), i1 + r4));
-            const MT us = (MT)(ui - u_ref), y0 = (MT)(i0 - y_ref0), y1 = (MT)(i1 - y_ref1),
+            const MT us = BX ? (MT)u : (MT)(ui - u_ref), y0 = (MT)(i0 - y_ref0), y1 = (MT)(i1 - y_ref1),
                      y2 = (MT)(i2 - y_ref2);
-            su_f += us;
+            if constexpr (!BX) su_f += us;
             s0 += y0; s1 += y1; s2 += y2;
             su0 = fma_t(us, y0, su0); su1 = fma_t(us, y1, su1);
             su2 = fma_t(us, y2, su2);
@@ -486,11 +503,17 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
         } else if (lowvar) {
             if (!(-1.0f < thr)) res = (float)d;  // no grid point is scored (agree.hpp:158)
         } else {
-            // centred moments (FP64 from the FP32 sums)
-            const double su_s = su_f, d0 = s0, d1 = s1, d2 = s2;
-            const double u0 = su0 - su_s * d0 * inv, u1 = su1 - su_s * d1 * inv, u2 = su2 - su_s * d2 * inv;
-            const double g00 = s00 - d0 * d0 * inv, g01 = s01 - d0 * d1 * inv, g02 = s02 - d0 * d2 * inv;
-            const double g11 = s11 - d1 * d1 * inv, g12 = s12 - d1 * d2 * inv, g22 = s22 - d2 * d2 * inv;
+            // centred moments: FP64 from the FP32 sums; BX: n times each is
+            // an exact integer in FP64 (n su0 - su s0, ...), rounded once by
+            // the division
+            const double su_s = BX ? su : (double)su_f, d0 = s0, d1 = s1, d2 = s2;
+            const auto centred = [&](double sxy, double sx, double sy) {
+                if constexpr (BX) return __builtin_fma((double)n.get(), sxy, -sx * sy) * inv;
+                else return sxy - sx * sy * inv;
+            };
+            const double u0 = centred(su0, su_s, d0), u1 = centred(su1, su_s, d1), u2 = centred(su2, su_s, d2);
+            const double g00 = centred(s00, d0, d0), g01 = centred(s01, d0, d1), g02 = centred(s02, d0, d2);
+            const double g11 = centred(s11, d1, d1), g12 = centred(s12, d1, d2), g22 = centred(s22, d2, d2);
             const float ua = (float)(0.5 * (u0 - 2.0 * u1 + u2)), ub = (float)(0.5 * (u2 - u0)), uc = (float)u1;
             const float aa = (float)(0.25 * (g00 + 4.0 * g11 + g22 - 4.0 * g01 + 2.0 * g02 - 4.0 * g12));
             const float ab = (float)(0.25 * (g22 - g00 - 2.0 * g12 + 2.0 * g01));

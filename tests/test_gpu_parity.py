@@ -843,7 +843,10 @@ def closed_form_decisions(oracle, L, R, raw, thr, step, minvar_n, margin):
     # border columns: the plain correlation at col1, which is the closed form at x = 0
     k0 = int(np.argmin(np.abs(grid)))
     scores = np.where(var1 < minvar_n, -1.0, nxc) if minvar_n is not None else nxc
-    scores = np.where(np.isnan(scores), -np.inf, scores)    # 0 / 0 never beats the best
+    # 0 / 0 never beats the best; nor a right variance that is zero up to
+    # rounding (below 2^-30 of the three columns' total variance)
+    total = var1[:, 0] + var1[:, int(np.argmin(np.abs(grid)))] + var1[:, -1]
+    scores = np.where(np.isnan(scores) | (var1 <= 2.0 ** -30 * total[:, None]), -np.inf, scores)
     low0 = (var0 < minvar_n) if minvar_n is not None else np.zeros(len(var0), bool)
     best_k = np.argmax(scores, axis=1)                      # first maximum
     best = scores[np.arange(len(best_k)), best_k]
@@ -883,16 +886,17 @@ def test_moments_fast_follows_closed_form(gpu, oracle, which):
         thr, step, mv, full = 0.9, 0.25, None, True
     n = L.shape[0]
     raw = oracle.search(oracle.transform(L, full), oracle.transform(R, full), 1)
-    # margin 1e-4: far above the FP32 evaluation error (coefficients rounded
-    # once to FP32, a degree-4 Horner scheme and rsq, a few 1e-7 relative)
-    exp, robust = closed_form_decisions(oracle, L, R, raw, thr, step, None if mv is None else mv * n, 1e-4)
+    # margin 1e-5: far above the kernel's evaluation error (the closed form
+    # in FP64 from exact integer coefficients, then two conversions to FP32,
+    # rsq and two products: a few 1e-7 relative)
+    exp, robust = closed_form_decisions(oracle, L, R, raw, thr, step, None if mv is None else mv * n, 1e-5)
     cfg = gpu.Config(nxcorr_threshold=thr, subpixel_step=step, min_variance=mv,
                      mode=gpu.TransformMode.Full if full else gpu.TransformMode.Limited,
                      precision=gpu.Precision.MomentsFast)
     got = host(gpu.match_device(dev(L), dev(R), cfg)[0])
     valid = np.isfinite(exp)
     print(f"MOMENTS_FAST {which}: {robust.mean():.4f} robust, {valid.mean():.3f} valid")
-    assert robust.mean() > 0.75 and (valid & robust).sum() > 0.75 * valid.sum()
+    assert robust.mean() > 0.9 and (valid & robust).sum() > 0.9 * valid.sum()
     bad = robust & ~(np.isnan(got) & np.isnan(exp) | (got == exp))
     assert not bad.any(), (int(bad.sum()), np.argwhere(bad)[:5].tolist())
 
