@@ -589,25 +589,55 @@ constexpr int CNN_FT_COLS = 34;             // stored cells per region row (32 +
 constexpr int CNN_FT_STRIDE = 40;           // BF16 values per stored cell (32 channels, 16 B pad)
 constexpr int CNN_FT_THREADS = 512;
 
+// The 16-byte chunks of one 3x3x32 layer's weights (wm layout [co][tap][ci])
+// that a thread of the workgroup stages at the start of their layer. Chunk
+// c = co * 36 + 4 tap + 2 h + g (channels 16 h + 8 g .. + 7 of the tap) is
+// the A operand of lane co of lane half g at K step s = 2 tap + h; it is
+// staged at (c % 36) * 32 + co = (2 s + g) * 32 + co, so that the 64 lanes of
+// each step read 64 consecutive chunks (no LDS bank conflicts).
+constexpr int CNN_WQ = CNN_C * 9 * CNN_C / 8;  // chunks per layer
+static_assert(CNN_WQ > 2 * CNN_FT_THREADS && CNN_WQ <= 3 * CNN_FT_THREADS, "three chunks per thread");
+struct CnnWeightChunks {  // this thread's chunks tid, tid + 512, tid + 1024 (the last if < CNN_WQ)
+    uint4 c0, c1, c2;
+};
+__device__ __forceinline__ void cnn_ft_weights_load(const uint16_t* __restrict__ wl, CnnWeightChunks& w) {
+    // unconditional (clamped) loads, so that nothing waits for them where they are issued
+    const uint4* q = (const uint4*)wl;
+    w.c0 = q[threadIdx.x];
+    w.c1 = q[threadIdx.x + CNN_FT_THREADS];
+    w.c2 = q[min((int)threadIdx.x + 2 * CNN_FT_THREADS, CNN_WQ - 1)];
+}
+__device__ __forceinline__ void cnn_ft_weights_stage(const CnnWeightChunks& w, uint4* __restrict__ wtmp) {
+    const auto put = [&](int c, const uint4& v) { wtmp[(c % 36) * 32 + c / 36] = v; };
+    put(threadIdx.x, w.c0);
+    put(threadIdx.x + CNN_FT_THREADS, w.c1);
+    if ((int)threadIdx.x + 2 * CNN_FT_THREADS < CNN_WQ) put(threadIdx.x + 2 * CNN_FT_THREADS, w.c2);
+}
+
 // one 3x3x32 layer over the stored region `in` (origin one cell up-left of
 // the output region's), rows / width of the output region; TO_LDS: into the
-// stored region `out`, else into the in-grid cells of the feature map y
-// The layer's weights are staged through `wtmp` (LDS the layer does not
-// read and has not yet written) once per workgroup, and each wave takes
-// its operands from there.
+// stored region `out`, else into the in-grid cells of the feature map y.
+// The layer's weights wl are staged through `wtmp` (LDS the layer does not
+// read and has not yet written) once per workgroup, and each wave takes its
+// operands from there (holding the next layer's chunks in registers during
+// the layer would cost the kernel a wave per SIMD).
 template<bool RELU, bool TO_LDS>
 __device__ __forceinline__ void cnn_ft_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
                                              int rows, int width, int oy, int ox, int Hq, int Wq,
-                                             const uint16_t* __restrict__ wl, const float* __restrict__ bias,
-                                             uint16_t* __restrict__ y, uint4* __restrict__ wtmp) {
+                                             const uint16_t* __restrict__ wl,
+                                             const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                             uint4* __restrict__ wtmp) {
     const int lane = threadIdx.x & 63, nl = lane & 31, g = lane >> 5, wave = threadIdx.x >> 6;
-    constexpr int WQ = CNN_C * 9 * CNN_C / 8;  // 16-byte chunks of the layer's weights
-    for (int i = threadIdx.x; i < WQ; i += CNN_FT_THREADS) wtmp[i] = ((const uint4*)wl)[i];
+    {
+        CnnWeightChunks w;
+        cnn_ft_weights_load(wl, w);
+        cnn_ft_weights_stage(w, wtmp);
+    }
     __syncthreads();
     bf16x8 wa[18];
 #pragma unroll
-    for (int s = 0; s < 18; ++s)
-        wa[s] = __builtin_bit_cast(bf16x8, wtmp[((nl * 9 + (s >> 1)) * CNN_C + 16 * (s & 1) + 8 * g) / 8]);
+    for (int s = 0; s < 18; ++s)  // chunk nl * 36 + 4 tap + 2 (s & 1) + g, tap = s >> 1
+        wa[s] = __builtin_bit_cast(bf16x8, wtmp[(2 * s + g) * 32 + nl]);
     __syncthreads();
     float bv[16];
 #pragma unroll
@@ -650,7 +680,7 @@ __device__ __forceinline__ void cnn_ft_layer(const uint16_t* __restrict__ in, ui
     }
 }
 
-__global__ __launch_bounds__(CNN_FT_THREADS) void k_cnn_features_fused(
+__global__ __launch_bounds__(CNN_FT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_cnn_features_fused(
     const uint16_t* __restrict__ xa, const uint16_t* __restrict__ xb, const uint16_t* __restrict__ w0,
     const uint16_t* __restrict__ wm, const float* __restrict__ params, int Hq, int Wq,
     uint16_t* __restrict__ ya, uint16_t* __restrict__ yb) {
@@ -666,12 +696,21 @@ __global__ __launch_bounds__(CNN_FT_THREADS) void k_cnn_features_fused(
     const int X0 = (blockIdx.x % tiles_x) * CNN_FT_W, Y0 = (blockIdx.x / tiles_x) * CNN_FT_H;
     const uint16_t* x = blockIdx.z ? xb : xa;
     uint16_t* y = blockIdx.z ? yb : ya;
-    for (int i = threadIdx.x; i < XR * CNN_FT_COLS; i += CNN_FT_THREADS) {
+    const size_t wl = (size_t)CNN_C * 9 * CNN_C;
+    // all loads in flight together: clamped addresses, zero selected outside the grid
+    constexpr int XIN_ITERS = (XR * CNN_FT_COLS + CNN_FT_THREADS - 1) / CNN_FT_THREADS;
+    uint2 xv[XIN_ITERS];
+#pragma unroll
+    for (int q = 0; q < XIN_ITERS; ++q) {
+        const int i = threadIdx.x + q * CNN_FT_THREADS;
         const int yy = Y0 - 4 + i / CNN_FT_COLS, xx = X0 - 4 + i % CNN_FT_COLS;
-        uint2 v = make_uint2(0, 0);
-        if (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq)
-            v = *(const uint2*)(x + ((size_t)yy * Wq + xx) * CNN_IN);
-        xin[i] = v;
+        const uint2 v = *(const uint2*)(x + ((size_t)min(max(yy, 0), Hq - 1) * Wq + min(max(xx, 0), Wq - 1)) * CNN_IN);
+        xv[q] = (yy >= 0 && yy < Hq && xx >= 0 && xx < Wq) ? v : make_uint2(0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < XIN_ITERS; ++q) {
+        const int i = threadIdx.x + q * CNN_FT_THREADS;
+        if (i < XR * CNN_FT_COLS) xin[i] = xv[q];
     }
     __syncthreads();
     {   // layer 0 (4 -> 32, ReLU) into A: rows Y0-3.., cols X0-3.. (k_cnn_conv0_mfma's products)
@@ -679,7 +718,12 @@ __global__ __launch_bounds__(CNN_FT_THREADS) void k_cnn_features_fused(
 #pragma unroll
         for (int st = 0; st < CNN_K0 / 16; ++st)
             wa[st] = __builtin_bit_cast(bf16x8, *(const uint4*)(w0 + nl * CNN_K0 + 16 * st + 8 * g));
-        const float* bias = params + cnn_b_off(0);
+        float bv[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 b4 = *(const float4*)(params + cnn_b_off(0) + 8 * q + 4 * g);
+            bv[4 * q] = b4.x; bv[4 * q + 1] = b4.y; bv[4 * q + 2] = b4.z; bv[4 * q + 3] = b4.w;
+        }
         for (int r = wave; r < A_ROWS; r += CNN_FT_THREADS / 64) {
             f32x16 acc = {0};
 #pragma unroll
@@ -699,18 +743,17 @@ __global__ __launch_bounds__(CNN_FT_THREADS) void k_cnn_features_fused(
                 float o[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    o[k] = inside ? fmaxf(acc[4 * q + k] + bias[8 * q + 4 * g + k], 0.0f) : 0.0f;
+                    o[k] = inside ? fmaxf(acc[4 * q + k] + bv[4 * q + k], 0.0f) : 0.0f;
                 *(uint2*)(dst + 8 * q) = make_uint2((uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16),
                                                     (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16));
             }
         }
     }
     __syncthreads();
-    const size_t wl = (size_t)CNN_C * 9 * CNN_C;
     // weights staged in the buffer each layer writes (B, A) or no longer
     // needs (B, after a3)
-    cnn_ft_layer<true, true>(bufA, bufB, B_ROWS, CNN_FT_W + 4, Y0 - 2, X0 - 2, Hq, Wq, wm, params + cnn_b_off(1),
-                             nullptr, bufB_raw);
+    cnn_ft_layer<true, true>(bufA, bufB, B_ROWS, CNN_FT_W + 4, Y0 - 2, X0 - 2, Hq, Wq, wm,
+                             params + cnn_b_off(1), nullptr, bufB_raw);
     __syncthreads();
     cnn_ft_layer<true, true>(bufB, bufA, CNN_FT_H + 2, CNN_FT_W + 2, Y0 - 1, X0 - 1, Hq, Wq, wm + wl,
                              params + cnn_b_off(2), nullptr, bufA_raw);

@@ -236,9 +236,9 @@ int cnn_windows(const Stack& v0, const Stack& v1, int depth, const Grid& g, floa
 
 /* --------------------------------------------------------- side stream */
 
-// Per device: a second stream (default priority) on which the network runs
-// while the descriptor transform runs on the caller's stream, and the fork /
-// join events. Guarded by g_ws_mu (held for the whole call).
+// Per device: a second stream (default priority) on which the descriptor
+// transform runs while the network runs on the caller's stream, and the
+// fork / join events. Guarded by g_ws_mu (held for the whole call).
 struct SideStream {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
@@ -716,8 +716,11 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         Stack v0, v1;
         if ((r = make_stack(f0, n, i0, pack0, &v0, s))) return r;
         if ((r = make_stack(f1, n, i1, pack1, &v1, s))) return r;
-        // the network (reads the frames only) on the side stream while the
-        // transform runs on the caller's
+        // the network (reads the frames only) on the caller's stream, the
+        // descriptor transform beside it on the side stream. The network is
+        // the longer of the two: it is submitted first, and the caller's
+        // stream, which goes on with the windowed pass, finds the shorter
+        // transform done when it reaches the join.
         SideStream* ss;
         if ((r = side_stream(dev, &ss))) return r;
         HIP_TRY(hipEventRecord(ss->fork, s));
@@ -727,16 +730,16 @@ int bicos_hip_match_windowed(const bicos_image* stack0, const bicos_image* stack
         const bool clear = fast && fallback;
         int rc_zero = BICOS_OK;
         if (clear && g_cnn_impl == BICOS_CNN_VALU)
-            rc_zero = launch("fallback_list", k_zero_u32, dim3((H + 255) / 256), dim3(256), 0, ss->stream,
+            rc_zero = launch("fallback_list", k_zero_u32, dim3((H + 255) / 256), dim3(256), 0, s,
                              (unsigned int*)row_count, H);
         const int rc_cnn = rc_zero ? rc_zero
                                    : cnn_windows(v0, v1, i0.depth, g, in_scale, params, opts, cnn_scratch,
-                                                 center, hw, nullptr, ss->stream,
+                                                 center, hw, nullptr, s,
                                                  clear && g_cnn_impl != BICOS_CNN_VALU ? row_count : nullptr, H);
+        r = rc_cnn ? rc_cnn
+                   : transform_any(v0, &v1, i0.depth, n, cfg->mode, W, H, words, desc0, desc1, ss->stream);
         // joined in every case: the workspace is released on the caller's stream
         HIP_TRY(hipEventRecord(ss->join, ss->stream));
-        r = rc_cnn ? rc_cnn
-                   : transform_any(v0, &v1, i0.depth, n, cfg->mode, W, H, words, desc0, desc1, s);
         HIP_TRY(hipStreamWaitEvent(s, ss->join, 0));
         if (r) return r;
         int16_t* d16 = subpix ? raw : (int16_t*)disparity->data;
