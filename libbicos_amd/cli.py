@@ -40,39 +40,38 @@ LICENSE_HEADER = ("libBICOS-compatible matching on AMD MI355X (libbicos_amd); "
 
 def parse_args(argv=None):
     p = argparse.ArgumentParser(prog="python -m libbicos_amd.cli",
-                                description="cli to process images with BICOS")
-    p.add_argument("folder0", help="First folder containing input images with numbered names.")
+                                description="BICOS stereo matching of two image stacks on the GPU")
+    p.add_argument("folder0", help="folder of numbered input images (N.png, or N_left.png and "
+                                   "N_right.png when folder1 is not given)")
     p.add_argument("folder1", nargs="?", default=None,
-                   help="Optional second folder with input images. If specified, file names need "
-                        "to be 0.png, 1.png... Else, folder0 needs to contain 0_left.png, "
-                        "0_right.png, 1_left.png...")
+                   help="folder of the second view's images; with it both folders hold "
+                        "0.png, 1.png, ...")
     p.add_argument("-t", "--threshold", type=float, default=0.75,
-                   help="Minimum normalized cross correlation for a match to be accepted. "
-                        "Set to 0.0 to disable.")
+                   help="smallest normalized cross-correlation a match keeps (0 turns the test off)")
     # shown as default 1.0, but (like cxxopts' count(), which does not count a
     # default value) min_variance is only set when -v is given: None here
     p.add_argument("-v", "--variance", type=float, default=None,
-                   help="Minimum intensity variance. Only active with --threshold. (default: 1.0, "
-                        "applied only when the option is given)")
+                   help="smallest intensity variance of a matched pixel, with --threshold only "
+                        "(default 1.0, applied only when the option is given)")
     p.add_argument("-s", "--step", type=float, default=None,
-                   help="Stepsize for subpixel interpolation. Only effective when threshold is set.")
-    p.add_argument("-o", "--out", default="bicosdisp.png", help="Output file for disparity image.")
+                   help="grid step of the subpixel refinement, with --threshold only")
+    p.add_argument("-o", "--out", default="bicosdisp.png", help="disparity image to write")
     p.add_argument("-n", "--stacksize", type=int, default=None,
-                   help="Number of images to process. Defaults to all found in the input folders.")
+                   help="how many images per view to use (default: every one found)")
     p.add_argument("-q", "--qmatrix", default=None,
-                   help="Path to cv::FileStorage with single matrix \"Q\" for reconstructing a pointcloud.")
+                   help="OpenCV FileStorage file holding the 4x4 matrix \"Q\"; writes a point cloud")
     p.add_argument("--allow-negative-z", action="store_true",
-                   help="Allow for points with negative Z values in the pointcloud output.")
+                   help="keep points with z < 0 in the point cloud")
     p.add_argument("-m", "--lr-maxdiff", type=int, default=None,
-                   help="Maximum disparity difference between left and right image. Enabling this "
-                        "disables duplicate filtering.")
-    p.add_argument("--double", action="store_true", help="Set double instead of single precision")
+                   help="largest left/right disparity difference (the Consistency variant; "
+                        "turns the duplicate filter off unless --no-dupes is given too)")
+    p.add_argument("--double", action="store_true", help="FP64 correlation instead of FP32")
     p.add_argument("--limited", action="store_true",
-                   help="Limit transformation mode. Allows for more images to be used.")
-    p.add_argument("--corrmap", action="store_true", help="Output map of normalized cross correlation values.")
+                   help="limited descriptor transform (shorter descriptors, deeper stacks)")
+    p.add_argument("--corrmap", action="store_true", help="also write the correlation map")
     p.add_argument("--no-dupes", action="store_true",
-                   help="Default BICOS variant when --lr-maxdiff is not specified. Can be set together "
-                        "with --lr-maxdiff to activate both.")
+                   help="the duplicate filter (the variant used without --lr-maxdiff; with it, "
+                        "both filters)")
     return p.parse_args(argv)
 
 

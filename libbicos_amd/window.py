@@ -325,7 +325,7 @@ def window_hits(center, hw, labels):
 
 
 def evaluate(params, spec=None, scenes: int = 200, opts: Optional[WindowOptions] = None,
-             precision: int = 3, seed0: int = EVAL_SEED0, device="cuda",
+             precision: int = 4, seed0: int = EVAL_SEED0, device="cuda",
              log: Optional[Callable[[str], None]] = print):
     """Learned vs plain path on `scenes` unseen scenes: agreement (share of
     plain-valid pixels that are valid in the learned path and within 0.5 px),
@@ -387,7 +387,7 @@ def _main(argv=None):
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
     e.add_argument("--scenes", type=int, default=200)
-    e.add_argument("--precision", type=int, default=3)
+    e.add_argument("--precision", type=int, default=4, help="3 MOMENTS, 4 MOMENTS_FAST (default)")
     e.add_argument("--hw-lo", type=int, default=None)
     e.add_argument("--hw-hi", type=int, default=None)
     e.add_argument("--p-conf", type=float, default=None)
