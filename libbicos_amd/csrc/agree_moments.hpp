@@ -221,11 +221,12 @@ __device__ void exact_pending_wave(uint64_t pend, int row, int col, int d, const
 }
 
 // Pixels the exact candidate stage takes per round of one wave (each brings
-// its candidate grid points, one lane per point, 64 lanes per round); fewer
-// for deep stacks, so that a workgroup's staging stays near 32 KiB of LDS.
+// its candidate grid points, one lane per point, 64 lanes per round). A wave
+// has a few pending pixels with one or two candidates each; 8 keeps the
+// staging loop short and a workgroup's LDS near 19 KiB at 33 frames.
 template<int NMAX>
 struct CandLds {
-    static constexpr int CAND_PIXELS = NMAX <= 36 ? 16 : 8;
+    static constexpr int CAND_PIXELS = 8;
     float4 xs[CAND_PIXELS * NMAX];  // (left sample, a, b, c) of frame t
     int row[CAND_PIXELS], col[CAND_PIXELS], c1[CAND_PIXELS], pre[CAND_PIXELS];
     double mean0[CAND_PIXELS];      // left mean in the correlation's precision
@@ -456,7 +457,7 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             // values and, where it is convex (concave) with the vertex
             // inside, the vertex, which lies above (below) y1 -/+ |y2 - y0| / 4
             // (rounded up here)
-            const int r4 = (abs(i2 - i0) + 3) >> 2;
+            const int r4 = (int)(__builtin_amdgcn_sad_u16((uint32_t)i2, (uint32_t)i0, 3u) >> 2);
             ylo = min(ylo, min(min(i0, i2), i1 - r4));
             yhi = max(yhi, max(max(i0, i2), i1 + r4));
             const MT us = BX ? (MT)u : (MT)(ui - u_ref), y0 = (MT)(i0 - y_ref0), y1 = (MT)(i1 - y_ref1),
@@ -580,13 +581,116 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             // error (cerr / sqrt(var0 var1) + verr / var1).
             const float slack = BX ? 1e-6f + 2.5e-7f * (float)n.get() : 1e-4f;
             const float dsq = delta * delta;
-            const auto bound = [&](float sv, float r1, float nxc) {
+            // the bound with w standing for sqrt(1 - r^2); it grows as sv
+            // falls and as r1 and w grow, every FP32 step being monotone in
+            // its non-negative operands
+            const auto bound_w = [&](float sv, float r1, float w) {
                 const float inv = 1.000001f * __builtin_amdgcn_rcpf(sv - delta);
-                // v_sqrt_f32 (1 ulp): w only scales the bound
-                const float w = 1.000001f * __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-nxc, nxc, 1.0005f), 0.0f));
                 return __builtin_fmaf(__builtin_fmaf(dsq, inv, w * delta), inv,
                                       __builtin_fmaf(r1, __builtin_fmaf(verr, r1, cerr_r0), slack));
             };
+            const auto bound = [&](float sv, float r1, float nxc) {
+                // v_sqrt_f32 (1 ulp): w only scales the bound
+                return bound_w(sv, r1, 1.000001f * __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-nxc, nxc, 1.0005f), 0.0f)));
+            };
+            bool full = true;  // the bound at every grid point (below)
+            if constexpr (BX) {
+                // Two levels. First the closed form alone at every point: its
+                // three largest values n1 >= n2 >= n3 (at points j1, j2), the
+                // smallest var1 and whether every |r| is usable. Then the
+                // bound at j1 and j2, and for every other point one bound
+                // taken at the smallest var1, which is at least that point's
+                // own: its sqrt(var1) is not below the smallest one (0.99999 of
+                // it is used, far beyond the rounding of v1 * rsq(v1)), its
+                // 1 / sqrt(var1) not above (1.00001 of it), and w taken at n3
+                // (below). The candidates are then j1, j2 if its upper bound
+                // reaches the larger lower bound of the two, and every point
+                // if n3's upper bound does. A pixel that misses a condition of the bound
+                // (sqrt(var1) > 2 delta, var1 clear of min_variance by the
+                // margin, |r| <= 1.0002, a left variance above zero, three grid
+                // points or more, so that j1 and j2 are points that take part)
+                // takes the per-point scan below.
+                // (v_med3 keeps the order statistics and the running minimum;
+                // the grid points come from the kernel arguments, x_k being
+                // the reference's running FP32 sum)
+                float n1 = -3.0f, n2 = -3.0f, n3 = -3.0f, vmin = __builtin_inff();
+                int j1 = 0, j2 = 0;
+                bool usable = true;
+                // At x = -1 the reference's interpolated samples are the
+                // column col1 - 1 itself (a - b + c = y0 exactly in FP32: a, b
+                // are half-integers below 2^17). Where that column is constant
+                // (g00 = 0, exact) its variance is 0 and the point never wins
+                // (-1 under min_variance, else 0 / 0); it takes no part here.
+                // (The closed form's error, proportional to the coefficients,
+                // cannot show a variance that small.)
+                {
+                    const float x = gx->x[0];
+                    const float v1 = v1_at(x);
+                    const float nxc = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0;
+                    if (g00 != 0.0) {
+                        vmin = v1;
+                        usable = fabsf(nxc) <= 1.0002f;
+                        n1 = nxc;
+                    }
+                }
+                for (int kk = 1; kk < npts; ++kk) {
+                    const float x = gx->x[kk];
+                    const float v1 = v1_at(x);
+                    const float nxc = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0;
+                    vmin = __builtin_amdgcn_fmed3f(vmin, v1, -__builtin_inff());
+                    usable = usable && fabsf(nxc) <= 1.0002f;  // false for NaN
+                    const bool g1 = nxc > n1, g2 = nxc > n2;
+                    j2 = g1 ? j1 : (g2 ? kk : j2);
+                    j1 = g1 ? kk : j1;
+                    n3 = __builtin_amdgcn_fmed3f(n2, n3, nxc);
+                    n2 = __builtin_amdgcn_fmed3f(n1, n2, nxc);
+                    n1 = __builtin_amdgcn_fmed3f(n1, nxc, __builtin_inff());
+                }
+                const float r1m = __builtin_amdgcn_rsqf(vmin), svm = vmin * r1m;
+                // var1 - margin(var1) rises with var1 where sqrt(var1) > delta, so
+                // the smallest var1 clearing min_variance by its margin (+1 for
+                // the rounding of the margin) clears it at every point
+                const bool clear_mv = !mv || vmin >= minvar + __builtin_fmaf(dd2, 1.00001f * svm,
+                                                                             __builtin_fmaf(1e-4f, vmin, dsq1)) + 1.0f;
+                if (settled && usable && npts >= 3 && var0d > 0.0 && vmin > 4.0004f * dsq && clear_mv) {
+                    full = false;
+                    const uint64_t all = npts >= 64 ? ~0ull : (1ull << npts) - 1ull;
+                    const auto at = [&](int j, float& hi) {  // lower bound at point j, its upper in hi
+                        const float x = gx->x[j];
+                        const float v1 = v1_at(x), r1 = __builtin_amdgcn_rsqf(v1);
+                        const float nxc = cov_at(x) * r1 * rv0;
+                        const float e = bound(v1 * r1, r1, nxc);
+                        hi = nxc + e;
+                        return nxc - e;
+                    };
+                    float hi1, hi2;
+                    const float lower2 = fmaxf(at(j1, hi1), at(j2, hi2));
+                    // Every other point has its value v <= n3, and its upper
+                    // bound is below v + a + c w(v) with c = delta inv, a the
+                    // rest of the bound and inv, r1 taken at the smallest var1.
+                    // v + c w(v) rises with v wherever c v < sqrt(1.0005 - v^2),
+                    // so if that holds at n3 (with room), w(n3) serves for all
+                    // of them (1.0002 covers the FP32 square roots); else w <=
+                    // 1.0003.
+                    const float svl = 0.99999f * svm;
+                    const float cw = 1.000001f * delta * (1.000001f * __builtin_amdgcn_rcpf(svl - delta));
+                    const float sq3 = __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-n3, n3, 1.0005f), 0.0f));
+                    const float w3 = cw * n3 <= 0.999f * sq3 ? 1.0002f * sq3 : 1.0003f;
+                    const float hir = n3 + bound_w(svl, 1.00001f * r1m, w3);
+                    if (!(fmaxf(fmaxf(fmaxf(hi1, hi2), hir), -1.0f) < thr)) {  // the reference's best starts at -1
+                        if (hi2 < lower2 && hir < lower2) {
+                            // one candidate: lower2 is j1's lower bound, its exact
+                            // value is at least that and above every other point's
+                            if (lower2 >= thr && lower2 > -1.0f) res = (float)d - gx->x[j1];
+                            else cmask = 1ull << j1;
+                        } else {
+                            cmask = hir < lower2 ? (1ull << j1) | (1ull << j2) : all;
+                        }
+                        if (cmask) st |= ST_PENDING;
+                    }
+                }
+            }
+            if (full) {
             for (float x = -1.0f; x <= 1.0f; x += step) {
                 const float cov = cov_at(x);
                 const float v1 = v1_at(x);
@@ -665,6 +769,7 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             } else if (!(settled && upper < thr)) {
                 st |= ST_PENDING;
             }
+            }  // full
         }
     }
     const uint64_t pend = __ballot(act && (st & ST_PENDING));
