@@ -157,16 +157,18 @@ __global__ __launch_bounds__(256) void k_subpixel_fast(const int16_t* __restrict
         lowvar);
 }
 
-// MOMENTS_FAST over the listed pixels of each row (block = row: entries
-// list[row * W + i], i < row_count[row]; the disparity of each is read from
-// disp at its column).
+// MOMENTS_FAST over the listed pixels of each row (entries list[row * W + i],
+// i < row_count[row]; the disparity of each is read from disp at its
+// column). One lane per pixel; block (row, b) takes entries b * 256 + j *
+// 256 * gridDim.y, j = 0, 1, ..., so that a row listing many pixels spreads
+// over several blocks instead of looping in one.
 template<typename TIn>
 __global__ __launch_bounds__(256) void k_subpixel_fast_list(
     const int16_t* __restrict__ disp, int64_t dstep, const Stack f0, const Stack f1, int n_rt, int W,
     float thr, int npts, const GridX gx, int has_mv, float minvar, float* __restrict__ out,
     int64_t ostep, const int16_t* __restrict__ list, const int* __restrict__ row_count) {
     const int row = blockIdx.x, count = row_count[row];
-    for (int i = threadIdx.x; i < count; i += 256) {
+    for (int i = blockIdx.y * 256 + threadIdx.x; i < count; i += gridDim.y * 256) {
         const int col = list[(size_t)row * W + i];
         const int d = disp[(size_t)row * dstep + col];
         if (d == INVALID16 || d == LISTED_SAME16) continue;  // output NaN already (the pixel is listed)

@@ -488,7 +488,8 @@ int run_subpixel_fast_list_t(const int16_t* disp, int64_t dstep, const Stack& f0
                              hipStream_t s) {
     const GridX gx = grid_x(step);
     const int npts = subpixel_points(step);
-    LAUNCH("agree_subpixel_fast_list", k_subpixel_fast_list<TIn>, dim3(H), dim3(256), 0, s,
+    // up to four blocks per row (256 listed pixels each per pass)
+    LAUNCH("agree_subpixel_fast_list", k_subpixel_fast_list<TIn>, dim3(H, 4), dim3(256), 0, s,
            disp, dstep, f0, f1, (int)n, W, thr, npts, gx, has_mv, mv, out, ostep, list, row_count);
     return BICOS_OK;
 }

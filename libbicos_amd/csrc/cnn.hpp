@@ -771,7 +771,7 @@ __global__ __launch_bounds__(CNN_FT_THREADS) __attribute__((amdgpu_waves_per_eu(
 constexpr int CNN_KMAX_MFMA = 160;
 
 // dynamic LDS of k_cnn_corr_window for K bins
-inline size_t cnn_corr_window_lds(int K) { return (size_t)4 * 32 * (K + 1) * sizeof(float); }
+inline size_t cnn_corr_window_lds(int K) { return (size_t)4 * (32 * (K + 1) + 1) * sizeof(float); }
 
 __global__ __launch_bounds__(256) void k_cnn_corr_window(const uint16_t* __restrict__ fl,
                                                          const uint16_t* __restrict__ fr, int Hq,
@@ -787,8 +787,7 @@ __global__ __launch_bounds__(256) void k_cnn_corr_window(const uint16_t* __restr
     if (Y >= Hq)
         return;
     const int ld = K + 1;
-    float* S = sc_dyn + (size_t)wv * 32 * ld;  // S[cell * ld + k]
-    for (int i = lane; i < 32 * K; i += 64) S[(i / K) * ld + i % K] = -1e30f;
+    float* S = sc_dyn + (size_t)wv * (32 * ld + 1);  // S[cell * ld + k], then a spare slot
     // left operands: cell X0 + nl, channels 16 h + 8 g .. +7
     const bf16x8 a0 = load_ch8(fl, Hq, Wq, Y, X0 + nl, 8 * g);
     const bf16x8 a1 = load_ch8(fl, Hq, Wq, Y, X0 + nl, 16 + 8 * g);
@@ -802,13 +801,20 @@ __global__ __launch_bounds__(256) void k_cnn_corr_window(const uint16_t* __restr
         f32x16 acc = {0};
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
-        if (Xr >= 0 && Xr < Wq) {
+        // every (cell, bin) entry comes up exactly once over the tiles:
+        // a right cell past the row gives -1e30 (no cell), which the scans
+        // below skip
+        // (entries outside [0, K) go to a spare slot past the wave's table,
+        // so the stores need no branches)
+        const bool in_row = Xr >= 0 && Xr < Wq;
+        const int kr = X0 - Xr - CNN_D0 + 4 * g;
+        const int rbase = 4 * g * ld + kr;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int m = 8 * (i / 4) + 4 * g + (i % 4);
-                const int k = (X0 + m) - Xr - CNN_D0;
-                if (k >= 0 && k < K) S[m * ld + k] = acc[i] * 0.17677669529663687f;  // 1 / sqrt(32)
-            }
+        for (int i = 0; i < 16; ++i) {
+            const int mi = 8 * (i / 4) + (i % 4);  // m - 4 g
+            const int k = kr + mi;
+            const int at = (unsigned)k < (unsigned)K ? rbase + mi * (ld + 1) : 32 * ld;
+            S[at] = in_row ? acc[i] * 0.17677669529663687f : -1e30f;  // 1 / sqrt(32)
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

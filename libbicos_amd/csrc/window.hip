@@ -325,9 +325,10 @@ int run_search_list_mfma4_w(const uint32_t* dq, const uint32_t* dc, int H, int W
                             int mode, const int16_t* qcol, const int* row_count, int16_t* res,
                             int16_t* disp, int64_t dstep, hipStream_t s) {
     constexpr size_t lds = (size_t)CH * (WORDS + 1) * 16;
-    // one block per row, looping over the row's groups of entries (a row
-    // lists a tenth of its pixels or so: mostly a single group)
-    const dim3 grid(H, 1), block(WAVES * 64);
+    // two blocks per row, taking alternate groups of entries (a row lists a
+    // tenth of its pixels or so: mostly a single group, which the second
+    // block leaves at once; rows listing more do not wait on one block)
+    const dim3 grid(H, 2), block(WAVES * 64);
 #define SM(ND, M) do { \
         int rc_ = allow_dynamic_lds((const void*)k_search_mfma_fp4<WORDS, NB, ND, M, WAVES, CH, true>, lds); \
         if (rc_) return rc_; \
