@@ -594,27 +594,33 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                 return bound_w(sv, r1, 1.000001f * __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-nxc, nxc, 1.0005f), 0.0f)));
             };
             bool full = true;  // the bound at every grid point (below)
-            if constexpr (BX) {
+            // x = -1 against a constant column col1 - 1 (below): never the maximum
+            const bool dead0 = BX && g00 == 0.0;
+            // Two levels pay where the grid is coarse; on finer grids (more
+            // than 32 points) the correlation curve is sampled so densely
+            // that its top points lie within each other's bounds and the
+            // per-point bound keeps fewer of them for the exact stage.
+            if (BX && npts <= 32) {
                 // Two levels. First the closed form alone at every point: its
-                // three largest values n1 >= n2 >= n3 (at points j1, j2), the
-                // smallest var1 and whether every |r| is usable. Then the
-                // bound at j1 and j2, and for every other point one bound
+                // four largest values n1 >= n2 >= n3 >= n4 (at points j1, j2,
+                // j3), the smallest var1 and whether every |r| is usable. Then
+                // the bound at j1, j2 and j3, and for every other point one bound
                 // taken at the smallest var1, which is at least that point's
                 // own: its sqrt(var1) is not below the smallest one (0.99999 of
                 // it is used, far beyond the rounding of v1 * rsq(v1)), its
-                // 1 / sqrt(var1) not above (1.00001 of it), and w taken at n3
-                // (below). The candidates are then j1, j2 if its upper bound
-                // reaches the larger lower bound of the two, and every point
-                // if n3's upper bound does. A pixel that misses a condition of the bound
-                // (sqrt(var1) > 2 delta, var1 clear of min_variance by the
-                // margin, |r| <= 1.0002, a left variance above zero, three grid
-                // points or more, so that j1 and j2 are points that take part)
-                // takes the per-point scan below.
+                // 1 / sqrt(var1) not above (1.00001 of it), and w taken at n4
+                // (below). The candidates are then j1, j2 and j3 where their
+                // upper bounds reach the largest lower bound of the three, and
+                // every point if n4's upper bound does. A pixel that misses a
+                // condition of the bound (sqrt(var1) > 2 delta, var1 clear of
+                // min_variance by the margin, |r| <= 1.0002, a left variance
+                // above zero, four grid points or more, so that j1, j2 and j3
+                // are points that take part) takes the per-point scan below.
                 // (v_med3 keeps the order statistics and the running minimum;
                 // the grid points come from the kernel arguments, x_k being
                 // the reference's running FP32 sum)
-                float n1 = -3.0f, n2 = -3.0f, n3 = -3.0f, vmin = __builtin_inff();
-                int j1 = 0, j2 = 0;
+                float n1 = -3.0f, n2 = -3.0f, n3 = -3.0f, n4 = -3.0f, vmin = __builtin_inff();
+                int j1 = 0, j2 = 0, j3 = 0;
                 bool usable = true;
                 // At x = -1 the reference's interpolated samples are the
                 // column col1 - 1 itself (a - b + c = y0 exactly in FP32: a, b
@@ -627,7 +633,7 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     const float x = gx->x[0];
                     const float v1 = v1_at(x);
                     const float nxc = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0;
-                    if (g00 != 0.0) {
+                    if (!dead0) {
                         vmin = v1;
                         usable = fabsf(nxc) <= 1.0002f;
                         n1 = nxc;
@@ -639,9 +645,11 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     const float nxc = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0;
                     vmin = __builtin_amdgcn_fmed3f(vmin, v1, -__builtin_inff());
                     usable = usable && fabsf(nxc) <= 1.0002f;  // false for NaN
-                    const bool g1 = nxc > n1, g2 = nxc > n2;
+                    const bool g1 = nxc > n1, g2 = nxc > n2, g3 = nxc > n3;
+                    j3 = g2 ? j2 : (g3 ? kk : j3);
                     j2 = g1 ? j1 : (g2 ? kk : j2);
                     j1 = g1 ? kk : j1;
+                    n4 = __builtin_amdgcn_fmed3f(n3, n4, nxc);
                     n3 = __builtin_amdgcn_fmed3f(n2, n3, nxc);
                     n2 = __builtin_amdgcn_fmed3f(n1, n2, nxc);
                     n1 = __builtin_amdgcn_fmed3f(n1, nxc, __builtin_inff());
@@ -652,7 +660,7 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                 // the rounding of the margin) clears it at every point
                 const bool clear_mv = !mv || vmin >= minvar + __builtin_fmaf(dd2, 1.00001f * svm,
                                                                              __builtin_fmaf(1e-4f, vmin, dsq1)) + 1.0f;
-                if (settled && usable && npts >= 3 && var0d > 0.0 && vmin > 4.0004f * dsq && clear_mv) {
+                if (settled && usable && npts >= 4 && var0d > 0.0 && vmin > 4.0004f * dsq && clear_mv) {
                     full = false;
                     const uint64_t all = npts >= 64 ? ~0ull : (1ull << npts) - 1ull;
                     const auto at = [&](int j, float& hi) {  // lower bound at point j, its upper in hi
@@ -663,28 +671,30 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                         hi = nxc + e;
                         return nxc - e;
                     };
-                    float hi1, hi2;
-                    const float lower2 = fmaxf(at(j1, hi1), at(j2, hi2));
-                    // Every other point has its value v <= n3, and its upper
+                    float hi1, hi2, hi3;
+                    const float lower = fmaxf(fmaxf(at(j1, hi1), at(j2, hi2)), at(j3, hi3));
+                    // Every other point has its value v <= n4, and its upper
                     // bound is below v + a + c w(v) with c = delta inv, a the
                     // rest of the bound and inv, r1 taken at the smallest var1.
                     // v + c w(v) rises with v wherever c v < sqrt(1.0005 - v^2),
-                    // so if that holds at n3 (with room), w(n3) serves for all
+                    // so if that holds at n4 (with room), w(n4) serves for all
                     // of them (1.0002 covers the FP32 square roots); else w <=
                     // 1.0003.
                     const float svl = 0.99999f * svm;
                     const float cw = 1.000001f * delta * (1.000001f * __builtin_amdgcn_rcpf(svl - delta));
-                    const float sq3 = __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-n3, n3, 1.0005f), 0.0f));
-                    const float w3 = cw * n3 <= 0.999f * sq3 ? 1.0002f * sq3 : 1.0003f;
-                    const float hir = n3 + bound_w(svl, 1.00001f * r1m, w3);
-                    if (!(fmaxf(fmaxf(fmaxf(hi1, hi2), hir), -1.0f) < thr)) {  // the reference's best starts at -1
-                        if (hi2 < lower2 && hir < lower2) {
-                            // one candidate: lower2 is j1's lower bound, its exact
+                    const float sq4 = __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-n4, n4, 1.0005f), 0.0f));
+                    const float w4 = cw * n4 <= 0.999f * sq4 ? 1.0002f * sq4 : 1.0003f;
+                    const float hir = n4 + bound_w(svl, 1.00001f * r1m, w4);
+                    if (!(fmaxf(fmaxf(fmaxf(fmaxf(hi1, hi2), hi3), hir), -1.0f) < thr)) {  // the reference's best starts at -1
+                        if (hir >= lower) {
+                            cmask = all;
+                        } else if (hi2 < lower && hi3 < lower) {
+                            // one candidate: lower is j1's lower bound, its exact
                             // value is at least that and above every other point's
-                            if (lower2 >= thr && lower2 > -1.0f) res = (float)d - gx->x[j1];
+                            if (lower >= thr && lower > -1.0f) res = (float)d - gx->x[j1];
                             else cmask = 1ull << j1;
                         } else {
-                            cmask = hir < lower2 ? (1ull << j1) | (1ull << j2) : all;
+                            cmask = (1ull << j1) | (hi2 >= lower ? 1ull << j2 : 0ull) | (hi3 >= lower ? 1ull << j3 : 0ull);
                         }
                         if (cmask) st |= ST_PENDING;
                     }
@@ -698,7 +708,9 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                 const float sv = v1 > 0.0f ? v1 * r1 : 0.0f;  // sqrt(var1)
                 const float margin = __builtin_fmaf(dd2, sv, __builtin_fmaf(1e-4f, fabsf(v1), dsq1));
                 float nxc = -1.0f, eps = 0.0f;
-                if (mv && v1 < minvar) {
+                if (dead0 && k == 0) {
+                    // x = -1 against a constant column: -1 or NaN, never the maximum
+                } else if (mv && v1 < minvar) {
                     // below the variance test: also exactly when below by the margin
                     if (v1 >= minvar - margin) settled = false;
                 } else {

@@ -191,13 +191,13 @@ def test_closed_form_from_moments_equals_definition(oracle, dtype, n):
 def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
     """The pruned kernel's two-level selection, restated in float64: the
     closed form alone at every point (x = -1 left out where the column
-    col1 - 1 is constant), the bound at the two largest values j1, j2, and
-    for every other point the bound at the smallest var1 with sqrt(1 - r^2)
-    taken at the third largest value n3. Returns (fast, cand, accept): fast
-    where the selection applies, cand (P, K) the candidate points, accept
-    where j1 alone is the result (clears thr by its bound). rest_scale and
-    pair_scale scale the bounds of the other points and of j1, j2 (1: the
-    kernel's; below 1: a planted mistake)."""
+    col1 - 1 is constant), the bound at the three largest values j1, j2, j3,
+    and for every other point the bound at the smallest var1 with
+    sqrt(1 - r^2) taken at the fourth largest value n4. Returns (fast, cand,
+    accept): fast where the selection applies, cand (P, K) the candidate
+    points, accept where j1 alone is the result (clears thr by its bound).
+    rest_scale and pair_scale scale the bounds of the other points and of
+    j1, j2, j3 (1: the kernel's; below 1: a planted mistake)."""
     P, K = r.shape
     d = 0.52 * np.sqrt(n)
     slack = 1e-6 + 2.5e-7 * n
@@ -208,37 +208,39 @@ def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
     vv[dead0, 0] = np.inf
     vmin = vv.min(1)
     live_r = np.where(np.isinf(vv), 0.0, rr)
-    fast = (vmin > 4.0004 * d * d) & (np.abs(live_r) <= 1.0002).all(1) & (K >= 3)
+    fast = (vmin > 4.0004 * d * d) & (np.abs(live_r) <= 1.0002).all(1) & (K >= 4)
     order = np.argsort(-rr, axis=1, kind="stable")
     ar = np.arange(P)
-    j1, j2 = order[:, 0], order[:, 1]
-    n3 = rr[ar, order[:, 2]]
+    js = order[:, :3]
+    n4 = rr[ar, order[:, 3]]
 
     def eps(sv, w):
         inv = 1.0 / (sv - d)
         return (d * d * inv + w * d) * inv + slack
 
     svm = np.sqrt(np.where(fast, vmin, 1e12))
-    e1 = eps(np.sqrt(np.where(fast, var1[ar, j1], 1e12)), np.sqrt(np.maximum(1.0005 - rr[ar, j1] ** 2, 0)))
-    e2 = eps(np.sqrt(np.where(fast, var1[ar, j2], 1e12)), np.sqrt(np.maximum(1.0005 - rr[ar, j2] ** 2, 0)))
+    e = [pair_scale * eps(np.sqrt(np.where(fast, var1[ar, j], 1e12)), np.sqrt(np.maximum(1.0005 - rr[ar, j] ** 2, 0)))
+         for j in js.T]
     c = d / (0.99999 * svm - d)
-    sq3 = np.sqrt(np.maximum(1.0005 - n3 * n3, 0))
-    w3 = np.where(c * n3 <= 0.999 * sq3, 1.0002 * sq3, 1.0003)
-    hir = n3 + rest_scale * eps(0.99999 * svm, w3)
-    e1, e2 = pair_scale * e1, pair_scale * e2
-    lower = np.maximum(rr[ar, j1] - e1, rr[ar, j2] - e2)
+    sq4 = np.sqrt(np.maximum(1.0005 - n4 * n4, 0))
+    w4 = np.where(c * n4 <= 0.999 * sq4, 1.0002 * sq4, 1.0003)
+    hir = n4 + rest_scale * eps(0.99999 * svm, w4)
+    lower = np.maximum.reduce([rr[ar, j] - ej for j, ej in zip(js.T, e)])
     cand = np.zeros((P, K), bool)
-    cand[ar, j1] = True
-    cand[ar, j2] |= rr[ar, j2] + e2 >= lower
+    cand[ar, js[:, 0]] = True
+    for j, ej in zip(js.T[1:], e[1:]):
+        cand[ar, j] |= rr[ar, j] + ej >= lower
     cand[hir >= lower] = True
     accept = (cand.sum(1) == 1) & (lower >= thr) & (lower > -1)
     return fast, cand, accept
 
 
-@pytest.mark.parametrize("dtype,n,step,P", [(np.uint16, 33, 0.1, 60_000), (np.uint16, 64, 0.05, 15_000),
+@pytest.mark.parametrize("dtype,n,step,P", [(np.uint16, 33, 0.1, 60_000), (np.uint16, 64, 0.1, 15_000),
                                             (np.uint8, 12, 0.25, 30_000)])
 def test_two_level_selection_keeps_the_reference_maximum(oracle, dtype, n, step, P):
-    """Wherever the two-level selection applies, the reference's first strict
+    """(The kernel takes the two-level selection on grids of at most 32
+    points; the 64-frame case runs it on the 20-point grid.) Wherever the
+    two-level selection applies, the reference's first strict
     maximum over the grid (its FP32 correlations, oracle.subpixel_curve) is
     one of the candidates, and a lone candidate accepted without the exact
     stage is that maximum and clears the threshold. Families as above, plus
@@ -282,11 +284,11 @@ def test_two_level_selection_keeps_the_reference_maximum(oracle, dtype, n, step,
 
 
 def test_two_level_selection_is_not_vacuous(oracle):
-    """The check above can fail: without the bound at j1 and j2 (their closed
-    forms taken as exact) the reference's maximum is lost, or a lone
+    """The check above can fail: without the bound at j1, j2 and j3 (their
+    closed forms taken as exact) the reference's maximum is lost, or a lone
     candidate is wrongly accepted, on correlated, near-threshold and
     saturated pixels. (The bound of the other points is not exercised by
-    these families: their third largest value lies well below the larger
+    these families: their fourth largest value lies well below the largest
     lower bound; it is kept because nothing rules the case out.)"""
     rng = np.random.default_rng(7300)
     n, step, thr = 33, 0.1, 0.96
