@@ -600,7 +600,11 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             // than 32 points) the correlation curve is sampled so densely
             // that its top points lie within each other's bounds and the
             // per-point bound keeps fewer of them for the exact stage.
-            if (BX && npts <= 32) {
+            // The choice is made per wave: where one of its pixels already
+            // fails a condition known before the scan (the parabolas' reach,
+            // a left variance above zero), the whole wave takes the per-point
+            // scan, which it would run anyway, instead of running both.
+            if (BX && npts <= 32 && __ballot(!(settled && var0d > 0.0)) == 0) {
                 // Two levels. First the closed form alone at every point: its
                 // four largest values n1 >= n2 >= n3 >= n4 (at points j1, j2,
                 // j3), the smallest var1 and whether every |r| is usable. Then
