@@ -372,8 +372,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
 // 4096 * dot, with dot = popcount(c) - 2 popcount(c & q) as above. All values
 // are integers below 2^24, so the FP32 accumulation is exact. Lane l of an
 // operand holds row (column) l % 32; descriptor word 2 s + g of K step s
-// goes to lane half g, bit 8 r + j to nibble j of register r (the same
-// packing for both operands, which is all the product needs).
+// goes to lane half g, its bit pair p (bits 2 p and 2 p + 1), p = 4 i + r,
+// to byte i of register r, the lower bit in the low nibble (the same packing
+// for both operands, which is all the product needs).
 //
 // Keys are FP32 values 2^20 + 4096 * dot + column (the accumulator starts
 // from 2^20 + the row within the tile), all positive, so their bit patterns
@@ -386,28 +387,28 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 constexpr int KEY4_BASE = 1 << 20;
 constexpr int KEY4_NONE = 0x4B000000;   // bit pattern of 2^23: above every real key
 
-// 8 bits -> 8 nibbles, bit j at 4 j + SH: even and odd bits separately, so
-// that no two partial products of a 24-bit multiply meet in a kept bit.
-template<int SH>
-__device__ __forceinline__ uint32_t spread8_nib(uint32_t byte) {
-    const uint32_t e = mul_u24(byte & 0x55u, 0x41041u << SH) & (0x01010101u << SH);
-    const uint32_t o = mul_u24(byte & 0xAAu, 0x208208u << SH) & (0x10101010u << SH);
-    return e | o;
+// 32 descriptor bits -> 32 fp4 entries in the packing above: v_perm_b32
+// looks every bit pair up in a table of 4 bytes (the selector byte is the
+// pair's value 0..3, which picks byte 0..3 of the table), 11 VALU
+// instructions per word.
+template<uint32_t TABLE>
+__device__ __forceinline__ uint4 expand_fp4(uint32_t w) {
+    constexpr uint32_t M = 0x03030303u;
+    return make_uint4(__builtin_amdgcn_perm(TABLE, TABLE, w & M),
+                      __builtin_amdgcn_perm(TABLE, TABLE, (w >> 2) & M),
+                      __builtin_amdgcn_perm(TABLE, TABLE, (w >> 4) & M),
+                      __builtin_amdgcn_perm(TABLE, TABLE, (w >> 6) & M));
 }
 
-// 32 candidate bits -> 32 fp4 entries 1.0 / 0
-__device__ __forceinline__ uint4 expand_c_fp4(uint32_t w) {
-    return make_uint4(spread8_nib<1>(w & 0xFFu), spread8_nib<1>((w >> 8) & 0xFFu),
-                      spread8_nib<1>((w >> 16) & 0xFFu), spread8_nib<1>(w >> 24));
-}
+// candidate bits -> 1.0 / 0: pair values 0, 1, 2, 3 -> bytes 0x00, 0x02,
+// 0x20, 0x22
+__device__ __forceinline__ uint4 expand_c_fp4(uint32_t w) { return expand_fp4<0x22200200u>(w); }
 
-// 32 query bits -> 32 fp4 entries +1.0 (bit 0) / -1.0 (bit 1)
+// query bits -> +1.0 (bit 0) / -1.0 (bit 1): pair values 0, 1, 2, 3 ->
+// bytes 0x22, 0x2A, 0xA2, 0xAA
 __device__ __forceinline__ v8i expand_q_fp4(uint32_t w) {
-    v8i r = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        r[k] = (int)((spread8_nib<0>((w >> (8 * k)) & 0xFFu) << 3) | 0x22222222u);
-    return r;
+    const uint4 v = expand_fp4<0xAAA22A22u>(w);
+    return v8i{(int)v.x, (int)v.y, (int)v.z, (int)v.w, 0, 0, 0, 0};
 }
 
 __device__ __forceinline__ int f_sub32(int key) {
