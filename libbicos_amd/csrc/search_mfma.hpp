@@ -368,23 +368,29 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
 // v_mfma_scale_f32_32x32x64_f8f6f4 with both operands in fp4 (E2M1) runs at
 // twice the int8 rate and covers 64 descriptor bits per product. Candidate
 // bits become 1.0 / 0 (codes 0b0010 / 0), query bits +1.0 / -1.0 (codes
-// 0b0010 / 0b1010); the E8M0 block scales 2^6 and 2^6 make the product
-// 4096 * dot, with dot = popcount(c) - 2 popcount(c & q) as above. All values
-// are integers below 2^24, so the FP32 accumulation is exact. Lane l of an
+// 0b0010 / 0b1010); the E8M0 block scales 2^7 (candidates) and 2^6
+// (queries) make the product 8192 * dot, with dot = popcount(c) - 2
+// popcount(c & q) as above. All values are integers below 2^24, so the FP32
+// accumulation is exact. Lane l of an
 // operand holds row (column) l % 32; descriptor word 2 s + g of K step s
 // goes to lane half g, its bit pair p (bits 2 p and 2 p + 1), p = 4 i + r,
 // to byte i of register r, the lower bit in the low nibble (the same packing
 // for both operands, which is all the product needs).
 //
-// Keys are FP32 values 2^20 + 4096 * dot + column (the accumulator starts
-// from 2^20 + the row within the tile), all positive, so their bit patterns
-// order like the values and the integer v_med3_i32 / v_min3_i32 reduction is
-// unchanged. The tile offset of the running state is an exact FP32
-// subtraction of 32 per tile.
+// Keys are FP32 values 2^22 + 8192 * dot + column (the accumulator starts
+// from 2^22 + the row within the tile; |8192 dot| <= 2^21), all positive, so
+// their bit patterns order like the values and the integer v_med3_i32 /
+// v_min3_i32 reduction is unchanged. The running minimum is kept relative to
+// the current tile (an exact FP32 subtraction of 32 per tile), so that its
+// column comes out exact. The second smallest key is only asked for its cost
+// (the duplicate test) and is left in the frame of the tile it came from: its
+// column then reads up to 32 * 128 = 4096 too large, which stays below one
+// cost step of 8192, so no comparison between keys of different costs
+// changes and its cost comes out exact.
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int KEY4_BASE = 1 << 20;
+constexpr int KEY4_BASE = 1 << 22;
 constexpr int KEY4_NONE = 0x4B000000;   // bit pattern of 2^23: above every real key
 
 // 32 descriptor bits -> 32 fp4 entries in the packing above: v_perm_b32
@@ -423,7 +429,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
     static_assert(WORDS % 2 == 0, "fp4 form: 64-bit K steps");
     constexpr int KS = WORDS / 2;          // K steps of 64 bits
     constexpr int STRIDE = WORDS + 1;      // uint4 per staged candidate (16 B pad)
-    constexpr int SCALE = 127 + 6;         // E8M0 2^6 for both operands
+    constexpr int SCALE_C = 127 + 7;       // E8M0 2^7 for the candidates,
+    constexpr int SCALE_Q = 127 + 6;       // 2^6 for the queries
     extern __shared__ uint4 lds[];
 
     const int row = LIST ? blockIdx.x : blockIdx.y;
@@ -503,11 +510,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 acc[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aop[0], qop[b][0], crow_init, 4, 4,
-                                                                         0, SCALE, 0, SCALE);
+                                                                         0, SCALE_C, 0, SCALE_Q);
 #pragma unroll
                 for (int s = 1; s < KS; ++s)
                     acc[b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(aop[s], qop[b][s], acc[b], 4, 4,
-                                                                             0, SCALE, 0, SCALE);
+                                                                             0, SCALE_C, 0, SCALE_Q);
             }
         };
         for (int mt = 0; mt < nfull; ++mt) {
@@ -531,8 +538,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
                     }
                 }
                 if constexpr (NODUPES)
-                    s2[b] = f_sub32(imin3(imin3(s2[b], t[0], t[1]), imin3(t[2], t[3], t[4]),
-                                          imin3(t[5], t[6], t[7])));
+                    s2[b] = imin3(imin3(s2[b], t[0], t[1]), imin3(t[2], t[3], t[4]),
+                                  imin3(t[5], t[6], t[7]));
                 s1[b] = f_sub32(s1[b]);
             }
         }
@@ -550,12 +557,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
                     s1[b] = min(s1[b], key);
                 }
                 s1[b] = f_sub32(s1[b]);
-                if constexpr (NODUPES) s2[b] = f_sub32(s2[b]);
             }
         }
     }
 
-    // absolute keys 4096 * dot + column, then merge lanes l, l ^ 32
+    // absolute keys 8192 * dot + column (the second one's column read up to
+    // 4096 too large), then merge lanes l, l ^ 32
     const float off = (float)(32 * ntiles - KEY4_BASE);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -565,8 +572,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
         const int m1 = min(b1, o1);
         const int m2 = min(max(b1, o1), min(b2, o2));
         const int col = qc[b];
-        const bool valid_key = !(NODUPES && (m1 >> 12) == (m2 >> 12));
-        const int best = m1 & 4095;
+        const bool valid_key = !(NODUPES && (m1 >> 13) == (m2 >> 13));
+        const int best = m1 & 8191;
         if constexpr (LIST) {
             const int i = qbase + b * 32 + nl;
             if (g != 0 || i >= count)
