@@ -605,26 +605,41 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             // a left variance above zero), the whole wave takes the per-point
             // scan, which it would run anyway, instead of running both.
             if (BX && npts <= 32 && __ballot(!(settled && var0d > 0.0)) == 0) {
-                // Two levels. First the closed form alone at every point: its
-                // four largest values n1 >= n2 >= n3 >= n4 (at points j1, j2,
-                // j3), the smallest var1 and whether every |r| is usable. Then
-                // the bound at j1, j2 and j3, and for every other point one bound
-                // taken at the smallest var1, which is at least that point's
-                // own: its sqrt(var1) is not below the smallest one (0.99999 of
-                // it is used, far beyond the rounding of v1 * rsq(v1)), its
-                // 1 / sqrt(var1) not above (1.00001 of it), and w taken at n4
-                // (below). The candidates are then j1, j2 and j3 where their
-                // upper bounds reach the largest lower bound of the three, and
-                // every point if n4's upper bound does. A pixel that misses a
-                // condition of the bound (sqrt(var1) > 2 delta, var1 clear of
-                // min_variance by the margin, |r| <= 1.0002, a left variance
-                // above zero, four grid points or more, so that j1, j2 and j3
-                // are points that take part) takes the per-point scan below.
-                // (v_med3 keeps the order statistics and the running minimum;
-                // the grid points come from the kernel arguments, x_k being
-                // the reference's running FP32 sum)
-                float n1 = -3.0f, n2 = -3.0f, n3 = -3.0f, n4 = -3.0f, vmin = __builtin_inff();
-                int j1 = 0, j2 = 0, j3 = 0;
+                // Two levels. First the closed form alone at every point,
+                // ranked by integer keys: 2^24 times the value truncated
+                // toward zero, its low 5 bits replaced by the point (npts <=
+                // 32); the four largest keys k1 >= k2 >= k3 >= k4 (points j1,
+                // j2, j3 in their low bits; v_med3 keeps the order
+                // statistics), the smallest var1 and whether every |r| is
+                // usable. Every point but j1, j2, j3 has its value below n4,
+                // the value of k4 rounded up to the next multiple of 32 / 2^24.
+                // Then the bound at j1, j2 and j3, and for every other point
+                // one bound taken at the smallest var1, which is at least that
+                // point's own: its sqrt(var1) is not below the smallest one
+                // (0.99999 of it is used, far beyond the rounding of v1 *
+                // rsq(v1)), its 1 / sqrt(var1) not above (1.00001 of it), and
+                // w taken at n4 (below). The candidates are then j1, j2 and j3
+                // where their upper bounds reach the largest lower bound of the
+                // three, and every point if n4's upper bound does. A pixel that
+                // misses a condition of the bound (sqrt(var1) > 2 delta,
+                // var1 clear of min_variance by the margin, |r| <= 1.0002, a
+                // left variance above zero, four grid points or more, so that
+                // j1, j2 and j3 are points that take part) takes the per-point
+                // scan below. (The grid points come from the kernel arguments,
+                // x_k being the reference's running FP32 sum.)
+                constexpr float KS = 16777216.0f;  // 2^24
+                constexpr int KEY_NONE = -3 * (1 << 24);
+                const float rv0s = rv0 * KS;  // exact: a power of two
+                // v_cvt_i32_f32: toward zero, saturating, 0 for NaN (a NaN or
+                // saturated value also fails |r| <= 1.0002); v_bfi_b32 puts
+                // the point (uniform, a scalar register) in the low 5 bits
+                const auto key_of = [](float v, int k) {
+                    int key;
+                    asm("v_cvt_i32_f32 %0, %1\n\tv_bfi_b32 %0, 31, %2, %0" : "=&v"(key) : "v"(v), "s"(k));
+                    return key;
+                };
+                int k1 = KEY_NONE, k2 = KEY_NONE, k3 = KEY_NONE, k4 = KEY_NONE;
+                float vmin = __builtin_inff();
                 bool usable = true;
                 // At x = -1 the reference's interpolated samples are the
                 // column col1 - 1 itself (a - b + c = y0 exactly in FP32: a, b
@@ -636,28 +651,31 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                 {
                     const float x = gx->x[0];
                     const float v1 = v1_at(x);
-                    const float nxc = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0;
+                    const float nxs = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0s;
                     if (!dead0) {
                         vmin = v1;
-                        usable = fabsf(nxc) <= 1.0002f;
-                        n1 = nxc;
+                        usable = fabsf(nxs) <= 1.0002f * KS;
+                        k1 = key_of(nxs, 0);
                     }
                 }
                 for (int kk = 1; kk < npts; ++kk) {
                     const float x = gx->x[kk];
                     const float v1 = v1_at(x);
-                    const float nxc = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0;
-                    vmin = __builtin_amdgcn_fmed3f(vmin, v1, -__builtin_inff());
-                    usable = usable && fabsf(nxc) <= 1.0002f;  // false for NaN
-                    const bool g1 = nxc > n1, g2 = nxc > n2, g3 = nxc > n3;
-                    j3 = g2 ? j2 : (g3 ? kk : j3);
-                    j2 = g1 ? j1 : (g2 ? kk : j2);
-                    j1 = g1 ? kk : j1;
-                    n4 = __builtin_amdgcn_fmed3f(n3, n4, nxc);
-                    n3 = __builtin_amdgcn_fmed3f(n2, n3, nxc);
-                    n2 = __builtin_amdgcn_fmed3f(n1, n2, nxc);
-                    n1 = __builtin_amdgcn_fmed3f(n1, nxc, __builtin_inff());
+                    const float nxs = cov_at(x) * __builtin_amdgcn_rsqf(v1) * rv0s;  // 2^24 r
+                    // smallest var1 as an integer min of the bit patterns: it
+                    // orders non-negative values like floats, and a negative
+                    // (or NaN) var1 comes out negative or leaves |r| <= 1.0002
+                    // false, either of which sends the pixel to the per-point scan
+                    vmin = __int_as_float(min(__float_as_int(vmin), __float_as_int(v1)));
+                    usable = usable && fabsf(nxs) <= 1.0002f * KS;  // false for NaN
+                    const int key = key_of(nxs, kk);
+                    k4 = imed3(k3, k4, key);
+                    k3 = imed3(k2, k3, key);
+                    k2 = imed3(k1, k2, key);
+                    k1 = max(k1, key);
                 }
+                const int j1 = k1 & 31, j2 = k2 & 31, j3 = k3 & 31;
+                const float n4 = (float)((k4 & ~31) + 32) * (1.0f / KS);  // exact
                 const float r1m = __builtin_amdgcn_rsqf(vmin), svm = vmin * r1m;
                 // var1 - margin(var1) rises with var1 where sqrt(var1) > delta, so
                 // the smallest var1 clearing min_variance by its margin (+1 for

@@ -97,4 +97,9 @@ __device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
     return max(min(a, b), min(max(a, b), c));
 }
 
+__device__ __forceinline__ int imed3(int a, int b, int c) {
+    // lowered to v_med3_i32
+    return max(min(a, b), min(max(a, b), c));
+}
+
 }  // namespace bicos_hip

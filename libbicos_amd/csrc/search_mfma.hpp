@@ -86,11 +86,6 @@ __device__ __forceinline__ v4i expand_pm64(uint32_t b16) {
     return r;
 }
 
-__device__ __forceinline__ int imed3(int a, int b, int c) {
-    // lowered to v_med3_i32
-    return max(min(a, b), min(max(a, b), c));
-}
-
 __device__ __forceinline__ int imin3(int a, int b, int c) {
     // lowered to v_min3_i32
     return min(min(a, b), c);

@@ -191,13 +191,16 @@ def test_closed_form_from_moments_equals_definition(oracle, dtype, n):
 def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
     """The pruned kernel's two-level selection, restated in float64: the
     closed form alone at every point (x = -1 left out where the column
-    col1 - 1 is constant), the bound at the three largest values j1, j2, j3,
-    and for every other point the bound at the smallest var1 with
-    sqrt(1 - r^2) taken at the fourth largest value n4. Returns (fast, cand,
-    accept): fast where the selection applies, cand (P, K) the candidate
-    points, accept where j1 alone is the result (clears thr by its bound).
-    rest_scale and pair_scale scale the bounds of the other points and of
-    j1, j2, j3 (1: the kernel's; below 1: a planted mistake)."""
+    col1 - 1 is constant), ranked by the kernel's integer keys (2^24 r
+    truncated toward zero, its low 5 bits replaced by the point), the bound at
+    the points j1, j2, j3 of the three largest keys, and for every other
+    point the bound at the smallest var1 with sqrt(1 - r^2) taken at n4, the
+    fourth largest key's value rounded up to the next multiple of 32 / 2^24
+    (above every other point's r). Returns (fast, cand, accept): fast where
+    the selection applies, cand (P, K) the candidate points, accept where j1
+    alone is the result (clears thr by its bound). rest_scale and pair_scale
+    scale the bounds of the other points and of j1, j2, j3 (1: the kernel's;
+    below 1: a planted mistake)."""
     P, K = r.shape
     d = 0.52 * np.sqrt(n)
     slack = 1e-6 + 2.5e-7 * n
@@ -209,10 +212,11 @@ def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
     vmin = vv.min(1)
     live_r = np.where(np.isinf(vv), 0.0, rr)
     fast = (vmin > 4.0004 * d * d) & (np.abs(live_r) <= 1.0002).all(1) & (K >= 4)
-    order = np.argsort(-rr, axis=1, kind="stable")
+    keys = (np.trunc(np.nan_to_num(rr, nan=0.0) * 2.0 ** 24).astype(np.int64) & ~31) | np.arange(K)
+    order = np.argsort(-keys, axis=1, kind="stable")
     ar = np.arange(P)
     js = order[:, :3]
-    n4 = rr[ar, order[:, 3]]
+    n4 = ((keys[ar, order[:, 3]] & ~31) + 32) / 2.0 ** 24
 
     def eps(sv, w):
         inv = 1.0 / (sv - d)
