@@ -345,14 +345,17 @@ int run_search_list_mfma4_w(const uint32_t* dq, const uint32_t* dc, int H, int W
 
 // Full-row search of the listed pixels: the full search's kernels with one
 // 32-query tile per wave (a row lists a tenth of its pixels or so, and
-// smaller query groups keep more of a workgroup's waves busy).
+// smaller query groups keep more of a workgroup's waves busy). 128-bit rows
+// are staged in chunks of 480 columns (38 KB of LDS, three workgroups per
+// CU): with so few queries per row, the staging's latency is what a
+// workgroup waits on, and a third workgroup per CU hides more of it.
 int run_search_list(const uint32_t* dq, const uint32_t* dc, int H, int W, int words, bool nodupes,
                     int mode, const int16_t* qcol, const int* row_count, int16_t* res,
                     int16_t* disp, int64_t dstep, hipStream_t s) {
     if (g_search_impl == BICOS_SEARCH_MFMA && words >= 2 && W <= EMBED_MAX_W) {
         switch (words) {
             case 2: return run_search_list_mfma4_w<2, 1, 8, 1344>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
-            case 4: return run_search_list_mfma4_w<4, 1, 8, 960>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
+            case 4: return run_search_list_mfma4_w<4, 1, 8, 480>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
             case 8: return run_search_list_mfma4_w<8, 1, 8, 448>(dq, dc, H, W, nodupes, mode, qcol, row_count, res, disp, dstep, s);
             default: return fail(BICOS_ERR_INVALID_ARGUMENT, "descriptor words must be 1, 2, 4 or 8");
         }
