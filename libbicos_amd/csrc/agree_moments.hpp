@@ -596,49 +596,56 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
             bool full = true;  // the bound at every grid point (below)
             // x = -1 against a constant column col1 - 1 (below): never the maximum
             const bool dead0 = BX && g00 == 0.0;
-            // Two levels pay where the grid is coarse; on finer grids (more
-            // than 32 points) the correlation curve is sampled so densely
-            // that its top points lie within each other's bounds and the
-            // per-point bound keeps fewer of them for the exact stage.
+            // Two levels: TL_K points of the first level are bounded exactly,
+            // three for stacks of at most 36 frames (grids of at most 32
+            // points), six for deeper stacks (at most 64 points: on the finer
+            // grids these usually take, the curve's top points lie within each
+            // other's bounds, and six exact points keep the candidate sets
+            // small). Other pixels and grids take the per-point scan below.
             // The choice is made per wave: where one of its pixels already
             // fails a condition known before the scan (the parabolas' reach,
             // a left variance above zero), the whole wave takes the per-point
             // scan, which it would run anyway, instead of running both.
-            if (BX && npts <= 32 && __ballot(!(settled && var0d > 0.0)) == 0) {
-                // Two levels. First the closed form alone at every point,
-                // ranked by integer keys: 2^24 times the value truncated
-                // toward zero, its low 5 bits replaced by the point (npts <=
-                // 32); the four largest keys k1 >= k2 >= k3 >= k4 (points j1,
-                // j2, j3 in their low bits; v_med3 keeps the order
-                // statistics), the smallest var1 and whether every |r| is
-                // usable. Every point but j1, j2, j3 has its value below n4,
-                // the value of k4 rounded up to the next multiple of 32 / 2^24.
-                // Then the bound at j1, j2 and j3, and for every other point
-                // one bound taken at the smallest var1, which is at least that
+            constexpr int TL_K = NMAX > 36 ? 6 : 3;   // exact points of the first level
+            constexpr int TL_IB = NMAX > 36 ? 6 : 5;  // bits of a point's index in its key
+            constexpr int TL_IM = (1 << TL_IB) - 1;
+            if (BX && npts <= TL_IM + 1 && __ballot(!(settled && var0d > 0.0)) == 0) {
+                // First the closed form alone at every point, ranked by
+                // integer keys: 2^24 times the value truncated toward zero,
+                // its low TL_IB bits replaced by the point; the TL_K + 1
+                // largest keys (points j_0 .. j_(TL_K - 1) in the low bits of
+                // the first TL_K; v_med3 keeps the order statistics), the
+                // smallest var1 and whether every |r| is usable. Every other
+                // point has its value below nr, the value of the next key
+                // rounded up to the next multiple of 2^TL_IB / 2^24. Then the
+                // bound at the TL_K points, and for every other point one
+                // bound taken at the smallest var1, which is at least that
                 // point's own: its sqrt(var1) is not below the smallest one
                 // (0.99999 of it is used, far beyond the rounding of v1 *
                 // rsq(v1)), its 1 / sqrt(var1) not above (1.00001 of it), and
-                // w taken at n4 (below). The candidates are then j1, j2 and j3
-                // where their upper bounds reach the largest lower bound of the
-                // three, and every point if n4's upper bound does. A pixel that
-                // misses a condition of the bound (sqrt(var1) > 2 delta,
-                // var1 clear of min_variance by the margin, |r| <= 1.0002, a
-                // left variance above zero, four grid points or more, so that
-                // j1, j2 and j3 are points that take part) takes the per-point
-                // scan below. (The grid points come from the kernel arguments,
-                // x_k being the reference's running FP32 sum.)
+                // w taken at nr (below). The candidates are then the TL_K
+                // points whose upper bounds reach the largest lower bound
+                // among them, and every point if nr's upper bound does. A
+                // pixel that misses a condition of the bound (sqrt(var1) > 2
+                // delta, var1 clear of min_variance by the margin, |r| <=
+                // 1.0002, a left variance above zero, more than TL_K grid
+                // points, so that the TL_K points take part) takes the
+                // per-point scan below. (The grid points come from the kernel
+                // arguments, x_k being the reference's running FP32 sum.)
                 constexpr float KS = 16777216.0f;  // 2^24
                 constexpr int KEY_NONE = -3 * (1 << 24);
                 const float rv0s = rv0 * KS;  // exact: a power of two
                 // v_cvt_i32_f32: toward zero, saturating, 0 for NaN (a NaN or
                 // saturated value also fails |r| <= 1.0002); v_bfi_b32 puts
-                // the point (uniform, a scalar register) in the low 5 bits
-                const auto key_of = [](float v, int k) {
+                // the point (uniform, a scalar register) in the low TL_IB bits
+                const auto key_of = [&](float v, int k) {
                     int key;
-                    asm("v_cvt_i32_f32 %0, %1\n\tv_bfi_b32 %0, 31, %2, %0" : "=&v"(key) : "v"(v), "s"(k));
+                    asm("v_cvt_i32_f32 %0, %1\n\tv_bfi_b32 %0, %3, %2, %0" : "=&v"(key) : "v"(v), "s"(k), "i"(TL_IM));
                     return key;
                 };
-                int k1 = KEY_NONE, k2 = KEY_NONE, k3 = KEY_NONE, k4 = KEY_NONE;
+                int kq[TL_K + 1];
+#pragma unroll
+                for (int i = 0; i <= TL_K; ++i) kq[i] = KEY_NONE;
                 float vmin = __builtin_inff();
                 bool usable = true;
                 // At x = -1 the reference's interpolated samples are the
@@ -655,7 +662,7 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     if (!dead0) {
                         vmin = v1;
                         usable = fabsf(nxs) <= 1.0002f * KS;
-                        k1 = key_of(nxs, 0);
+                        kq[0] = key_of(nxs, 0);
                     }
                 }
                 for (int kk = 1; kk < npts; ++kk) {
@@ -669,20 +676,21 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                     vmin = __int_as_float(min(__float_as_int(vmin), __float_as_int(v1)));
                     usable = usable && fabsf(nxs) <= 1.0002f * KS;  // false for NaN
                     const int key = key_of(nxs, kk);
-                    k4 = imed3(k3, k4, key);
-                    k3 = imed3(k2, k3, key);
-                    k2 = imed3(k1, k2, key);
-                    k1 = max(k1, key);
+#pragma unroll
+                    for (int i = TL_K; i >= 1; --i) kq[i] = imed3(kq[i - 1], kq[i], key);
+                    kq[0] = max(kq[0], key);
                 }
-                const int j1 = k1 & 31, j2 = k2 & 31, j3 = k3 & 31;
-                const float n4 = (float)((k4 & ~31) + 32) * (1.0f / KS);  // exact
+                int js[TL_K];
+#pragma unroll
+                for (int i = 0; i < TL_K; ++i) js[i] = kq[i] & TL_IM;
+                const float nr = (float)((kq[TL_K] & ~TL_IM) + TL_IM + 1) * (1.0f / KS);  // exact
                 const float r1m = __builtin_amdgcn_rsqf(vmin), svm = vmin * r1m;
                 // var1 - margin(var1) rises with var1 where sqrt(var1) > delta, so
                 // the smallest var1 clearing min_variance by its margin (+1 for
                 // the rounding of the margin) clears it at every point
                 const bool clear_mv = !mv || vmin >= minvar + __builtin_fmaf(dd2, 1.00001f * svm,
                                                                              __builtin_fmaf(1e-4f, vmin, dsq1)) + 1.0f;
-                if (settled && usable && npts >= 4 && var0d > 0.0 && vmin > 4.0004f * dsq && clear_mv) {
+                if (settled && usable && npts >= TL_K + 1 && var0d > 0.0 && vmin > 4.0004f * dsq && clear_mv) {
                     full = false;
                     const uint64_t all = npts >= 64 ? ~0ull : (1ull << npts) - 1ull;
                     const auto at = [&](int j, float& hi) {  // lower bound at point j, its upper in hi
@@ -693,30 +701,42 @@ __device__ __forceinline__ void agree_subpixel_moments_px(
                         hi = nxc + e;
                         return nxc - e;
                     };
-                    float hi1, hi2, hi3;
-                    const float lower = fmaxf(fmaxf(at(j1, hi1), at(j2, hi2)), at(j3, hi3));
-                    // Every other point has its value v <= n4, and its upper
+                    float his[TL_K];
+                    float lower = -__builtin_inff();
+#pragma unroll
+                    for (int i = 0; i < TL_K; ++i) lower = fmaxf(lower, at(js[i], his[i]));
+                    // Every other point has its value v <= nr, and its upper
                     // bound is below v + a + c w(v) with c = delta inv, a the
                     // rest of the bound and inv, r1 taken at the smallest var1.
                     // v + c w(v) rises with v wherever c v < sqrt(1.0005 - v^2),
-                    // so if that holds at n4 (with room), w(n4) serves for all
+                    // so if that holds at nr (with room), w(nr) serves for all
                     // of them (1.0002 covers the FP32 square roots); else w <=
                     // 1.0003.
                     const float svl = 0.99999f * svm;
                     const float cw = 1.000001f * delta * (1.000001f * __builtin_amdgcn_rcpf(svl - delta));
-                    const float sq4 = __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-n4, n4, 1.0005f), 0.0f));
-                    const float w4 = cw * n4 <= 0.999f * sq4 ? 1.0002f * sq4 : 1.0003f;
-                    const float hir = n4 + bound_w(svl, 1.00001f * r1m, w4);
-                    if (!(fmaxf(fmaxf(fmaxf(fmaxf(hi1, hi2), hi3), hir), -1.0f) < thr)) {  // the reference's best starts at -1
+                    const float sqr = __builtin_amdgcn_sqrtf(fmaxf(__builtin_fmaf(-nr, nr, 1.0005f), 0.0f));
+                    const float wr = cw * nr <= 0.999f * sqr ? 1.0002f * sqr : 1.0003f;
+                    const float hir = nr + bound_w(svl, 1.00001f * r1m, wr);
+                    float hmax = fmaxf(hir, -1.0f);  // the reference's best starts at -1
+#pragma unroll
+                    for (int i = 0; i < TL_K; ++i) hmax = fmaxf(hmax, his[i]);
+                    if (!(hmax < thr)) {
                         if (hir >= lower) {
                             cmask = all;
-                        } else if (hi2 < lower && hi3 < lower) {
-                            // one candidate: lower is j1's lower bound, its exact
-                            // value is at least that and above every other point's
-                            if (lower >= thr && lower > -1.0f) res = (float)d - gx->x[j1];
-                            else cmask = 1ull << j1;
                         } else {
-                            cmask = (1ull << j1) | (hi2 >= lower ? 1ull << j2 : 0ull) | (hi3 >= lower ? 1ull << j3 : 0ull);
+                            uint64_t m = 1ull << js[0];
+                            bool more = false;
+#pragma unroll
+                            for (int i = 1; i < TL_K; ++i)
+                                if (his[i] >= lower) {
+                                    m |= 1ull << js[i];
+                                    more = true;
+                                }
+                            // one candidate: lower is j_0's lower bound (the point
+                            // attaining it is a candidate), its exact value is at
+                            // least that and above every other point's
+                            if (!more && lower >= thr && lower > -1.0f) res = (float)d - gx->x[js[0]];
+                            else cmask = m;
                         }
                         if (cmask) st |= ST_PENDING;
                     }

@@ -188,20 +188,23 @@ def test_closed_form_from_moments_equals_definition(oracle, dtype, n):
                 assert abs(r_def[i, 0] - oracle.nxcorrd(u[i], y[i, 0])) < 1e-12
 
 
-def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
+def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0, top=3):
     """The pruned kernel's two-level selection, restated in float64: the
     closed form alone at every point (x = -1 left out where the column
     col1 - 1 is constant), ranked by the kernel's integer keys (2^24 r
-    truncated toward zero, its low 5 bits replaced by the point), the bound at
-    the points j1, j2, j3 of the three largest keys, and for every other
-    point the bound at the smallest var1 with sqrt(1 - r^2) taken at n4, the
-    fourth largest key's value rounded up to the next multiple of 32 / 2^24
+    truncated toward zero, its low bits replaced by the point: 5 bits for the
+    three exact points of stacks up to 36 frames, 6 for the six of deeper
+    stacks), the bound at the points of the `top` largest keys, and for every
+    other point the bound at the smallest var1 with sqrt(1 - r^2) taken at
+    the next key's value rounded up to the next multiple of 2^bits / 2^24
     (above every other point's r). Returns (fast, cand, accept): fast where
-    the selection applies, cand (P, K) the candidate points, accept where j1
-    alone is the result (clears thr by its bound). rest_scale and pair_scale
-    scale the bounds of the other points and of j1, j2, j3 (1: the kernel's;
-    below 1: a planted mistake)."""
+    the selection applies, cand (P, K) the candidate points, accept where the
+    largest key's point alone is the result (clears thr by its bound).
+    rest_scale and pair_scale scale the bounds of the other points and of the
+    exact ones (1: the kernel's; below 1: a planted mistake)."""
     P, K = r.shape
+    ib = 5 if top == 3 else 6
+    unit = 1 << ib
     d = 0.52 * np.sqrt(n)
     slack = 1e-6 + 2.5e-7 * n
     dead0 = np.all(y[:, 0, :] == y[:, 0, :1], axis=1)
@@ -211,12 +214,12 @@ def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
     vv[dead0, 0] = np.inf
     vmin = vv.min(1)
     live_r = np.where(np.isinf(vv), 0.0, rr)
-    fast = (vmin > 4.0004 * d * d) & (np.abs(live_r) <= 1.0002).all(1) & (K >= 4)
-    keys = (np.trunc(np.nan_to_num(rr, nan=0.0) * 2.0 ** 24).astype(np.int64) & ~31) | np.arange(K)
+    fast = (vmin > 4.0004 * d * d) & (np.abs(live_r) <= 1.0002).all(1) & (K >= top + 1)
+    keys = (np.trunc(np.nan_to_num(rr, nan=0.0) * 2.0 ** 24).astype(np.int64) & ~(unit - 1)) | np.arange(K)
     order = np.argsort(-keys, axis=1, kind="stable")
     ar = np.arange(P)
-    js = order[:, :3]
-    n4 = ((keys[ar, order[:, 3]] & ~31) + 32) / 2.0 ** 24
+    js = order[:, :top]
+    n4 = ((keys[ar, order[:, top]] & ~(unit - 1)) + unit) / 2.0 ** 24
 
     def eps(sv, w):
         inv = 1.0 / (sv - d)
@@ -239,12 +242,14 @@ def two_level_candidates(r, var1, y, n, thr, rest_scale=1.0, pair_scale=1.0):
     return fast, cand, accept
 
 
-@pytest.mark.parametrize("dtype,n,step,P", [(np.uint16, 33, 0.1, 60_000), (np.uint16, 64, 0.1, 15_000),
-                                            (np.uint8, 12, 0.25, 30_000)])
-def test_two_level_selection_keeps_the_reference_maximum(oracle, dtype, n, step, P):
-    """(The kernel takes the two-level selection on grids of at most 32
-    points; the 64-frame case runs it on the 20-point grid.) Wherever the
-    two-level selection applies, the reference's first strict
+@pytest.mark.parametrize("dtype,n,step,P,top", [(np.uint16, 33, 0.1, 60_000, 3), (np.uint16, 64, 0.1, 15_000, 6),
+                                                (np.uint16, 64, 0.05, 15_000, 6), (np.uint8, 12, 0.25, 30_000, 3)])
+def test_two_level_selection_keeps_the_reference_maximum(oracle, dtype, n, step, P, top):
+    """(The kernel takes the two-level selection with three exact points on
+    grids of at most 32 points for stacks up to 36 frames, with six on grids
+    of at most 64 points for deeper ones: the 64-frame cases run it on the
+    20- and 40-point grids.) Wherever the two-level selection applies, the
+    reference's first strict
     maximum over the grid (its FP32 correlations, oracle.subpixel_curve) is
     one of the candidates, and a lone candidate accepted without the exact
     stage is that maximum and clears the threshold. Families as above, plus
@@ -263,7 +268,7 @@ def test_two_level_selection_keeps_the_reference_maximum(oracle, dtype, n, step,
         r, var1, var0 = exact_closed_form(u, y, grid)
         ref = oracle.subpixel_curve(u, y, step).astype(np.float64)
         use = reach_in_range(y, vmax) & (var0 > 0)
-        fast, cand, accept = two_level_candidates(r, var1, y, n, thr)
+        fast, cand, accept = two_level_candidates(r, var1, y, n, thr, top=top)
         fast &= use
         # the reference's first strict maximum, starting from -1 (agree.hpp:159-171)
         best = np.full(len(ref), -1.0)
