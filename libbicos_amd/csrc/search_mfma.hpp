@@ -140,6 +140,19 @@ constexpr int EMBED_MAX_W = 4096;
 // XCDs. Results go to out[r * ostep + col] (SEARCH_DISP) or to
 // res[r * W + i] (SEARCH_REV). Waves without listed queries skip the
 // arithmetic but take part in the staging.
+// Row and query group of a full-search workgroup (grid: groups per row x
+// rows). Workgroups go round-robin over the 8 XCDs by their linear index,
+// so the groups of one row are given consecutive slots of one XCD (where the
+// block count is a multiple of 8), and the row's candidate descriptors come
+// into that XCD's L2 once instead of once per group.
+__device__ __forceinline__ void xcd_row_group(int& row, int& group) {
+    const int nb = gridDim.x, total = nb * gridDim.y;
+    const int lin = blockIdx.x + nb * blockIdx.y;
+    const int q = total % 8 == 0 ? (lin % 8) * (total / 8) + lin / 8 : lin;
+    row = q / nb;
+    group = q - row * nb;
+}
+
 template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH, bool LIST = false,
          bool EMBED = false>
 __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
@@ -152,10 +165,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
     constexpr int CMASK = (1 << CSHIFT) - 1;
     extern __shared__ uint4 lds[];
 
-    const int row = LIST ? blockIdx.x : blockIdx.y;
+    int row, group;
+    if constexpr (LIST) {
+        row = blockIdx.x;
+        group = blockIdx.y;
+    } else {
+        xcd_row_group(row, group);
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nl = lane & 31, g = lane >> 5;
-    const int group = LIST ? blockIdx.y : blockIdx.x;
     int count = 0;
     if constexpr (LIST) {
         count = row_count[row];
@@ -428,13 +446,19 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
     constexpr int SCALE_Q = 127 + 6;       // 2^6 for the queries
     extern __shared__ uint4 lds[];
 
-    const int row = LIST ? blockIdx.x : blockIdx.y;
+    int row, group0;
+    if constexpr (LIST) {
+        row = blockIdx.x;
+        group0 = blockIdx.y;
+    } else {
+        xcd_row_group(row, group0);
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nl = lane & 31, g = lane >> 5;
     // LIST: a block serves groups blockIdx.y, blockIdx.y + gridDim.y, ...
     // of WAVES * NB * 32 entries of its row (staging the row again per group)
     const int count = LIST ? row_count[row] : 0;
-    for (int group = LIST ? blockIdx.y : blockIdx.x;; group += gridDim.y) {
+    for (int group = group0;; group += gridDim.y) {
     if constexpr (LIST) {
         if (group * WAVES * NB * 32 >= count)
             return;  // uniform over the block
