@@ -92,6 +92,20 @@ __device__ __forceinline__ void list_entry(const RowLists& l, int i, int& row, i
     col = l.cols[(size_t)lo * l.W + (i - l.row_start[lo])];
 }
 
+// Row and column group of a workgroup of a (groups per row) x (rows) grid.
+// Workgroups go round-robin over the 8 XCDs by their linear index, so the
+// groups of one row are given consecutive slots of one XCD (where the block
+// count is a multiple of 8): what they share of the row (candidate
+// descriptors, right-view columns) comes into that XCD's L2 once instead of
+// once per group.
+__device__ __forceinline__ void xcd_row_group(int& row, int& group) {
+    const int nb = gridDim.x, total = nb * gridDim.y;
+    const int lin = blockIdx.x + nb * blockIdx.y;
+    const int q = total % 8 == 0 ? (lin % 8) * (total / 8) + lin / 8 : lin;
+    row = q / nb;
+    group = q - row * nb;
+}
+
 __device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
     // lowered to v_med3_u32
     return max(min(a, b), min(max(a, b), c));

@@ -140,19 +140,6 @@ constexpr int EMBED_MAX_W = 4096;
 // XCDs. Results go to out[r * ostep + col] (SEARCH_DISP) or to
 // res[r * W + i] (SEARCH_REV). Waves without listed queries skip the
 // arithmetic but take part in the staging.
-// Row and query group of a full-search workgroup (grid: groups per row x
-// rows). Workgroups go round-robin over the 8 XCDs by their linear index,
-// so the groups of one row are given consecutive slots of one XCD (where the
-// block count is a multiple of 8), and the row's candidate descriptors come
-// into that XCD's L2 once instead of once per group.
-__device__ __forceinline__ void xcd_row_group(int& row, int& group) {
-    const int nb = gridDim.x, total = nb * gridDim.y;
-    const int lin = blockIdx.x + nb * blockIdx.y;
-    const int q = total % 8 == 0 ? (lin % 8) * (total / 8) + lin / 8 : lin;
-    row = q / nb;
-    group = q - row * nb;
-}
-
 template<int WORDS, int NB, bool NODUPES, int MODE, int WAVES, int CH, bool LIST = false,
          bool EMBED = false>
 __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
