@@ -450,6 +450,28 @@ def test_windowed_match_matches_oracle_4096_band(W, oracle):
     assert not (np.isnan(got) & np.isfinite(plain)).any()
 
 
+@pytest.mark.parametrize("variant", ["nodupes", "consistency"])
+def test_windowed_match_matches_oracle_u8_three_chunks(W, oracle, variant):
+    """12 frames u8 (64-bit descriptors) on 2689-column rows: the fallback
+    list search stages 64-bit candidate rows in chunks of 1344 columns, so a
+    row is two full chunks and one of a single column. An untrained
+    network's narrow windows send most pixels to the fallback; the learned
+    path equals the oracle's windowed match bit for bit."""
+    import libbicos_amd as B
+    from libbicos_amd import scene
+    assert B.descriptor_bits(12, 0) == 64
+    L, R, _ = W.scene_device(scene.SCENE_640x480_U8, 0xC0FFEE, height=8, width=2689)
+    params = W.params_device(W.init_params(5))
+    opts = W.WindowOptions(hw_lo=2, hw_hi=5, fallback=W.FALLBACK_ALL)
+    cfg_kw = dict(nxcorr_threshold=0.9, subpixel_step=0.25, min_variance=1.0)
+    okw = dict(thr=0.9, step=0.25, min_variance=1.0)
+    if variant == "consistency":
+        cfg_kw["variant"] = B.Variant.Consistency(max_lr_diff=2, no_dupes=True)
+        okw["consistency"] = (2, True)
+    got, st, nfb, _ = _windowed_vs_oracle(W, oracle, L, R, params, opts, cfg_kw, okw, corrmap=True)
+    assert got.shape == (8, 2689) and st["fallback_pixels"] == nfb > got.size // 2
+
+
 def test_windowed_errors(W):
     import torch
     import libbicos_amd as B
