@@ -871,7 +871,11 @@ __global__ __launch_bounds__(256) void k_agree_subpixel_pruned(
     int W, float thr, float step, int npts, int has_minvar, float minvar, float* __restrict__ out,
     int64_t ostep, const GridX gx) {
     __shared__ CandLds<NMAX> cl[4];
-    const int col = blockIdx.x * 256 + threadIdx.x, row = blockIdx.y;
+    // the workgroups of one row on consecutive slots of one XCD: its L2 then
+    // holds the right-view lines that neighbouring workgroups share
+    int row, bx;
+    xcd_row_group(row, bx);
+    const int col = bx * 256 + threadIdx.x;
     if ((col & ~63) >= W)  // whole wave past the row
         return;
     agree_subpixel_moments_px<TIn, NMAX, N_EXACT, true, TPrec>(col < W, row, min(col, W - 1), disp, dstep,
