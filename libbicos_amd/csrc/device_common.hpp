@@ -106,6 +106,24 @@ __device__ __forceinline__ void xcd_row_group(int& row, int& group) {
     group = q - row * nb;
 }
 
+// The same with the rows dealt out over the XCDs (XCD x takes rows x, x + 8,
+// ..., each row's groups on consecutive slots of its XCD; where the row count
+// is a multiple of 8): the row's shared data still comes into one L2, and
+// every XCD gets rows from every part of the image, so work that varies with
+// the image region stays balanced over the XCDs.
+__device__ __forceinline__ void xcd_row_group_il(int& row, int& group) {
+    const int nb = gridDim.x, H = gridDim.y;
+    const int lin = blockIdx.x + nb * blockIdx.y;
+    if (H % 8 == 0) {
+        const int x = lin % 8, j = lin / 8, rb = j / nb;
+        row = 8 * rb + x;
+        group = j - rb * nb;
+    } else {
+        row = blockIdx.y;
+        group = blockIdx.x;
+    }
+}
+
 __device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
     // lowered to v_med3_u32
     return max(min(a, b), min(max(a, b), c));

@@ -184,7 +184,9 @@ __global__ __launch_bounds__(256) void k_window_core(
     uint32_t* ldesc = rdesc + (size_t)span * WORDS;           // span x WORDS (Consistency)
     __shared__ int red[4][3], wsum[4], base_s;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row = blockIdx.y, c0 = blockIdx.x * 256;
+    int row, bx;
+    xcd_row_group_il(row, bx);  // one row's workgroups on one XCD, rows dealt out over the XCDs
+    const int c0 = bx * 256;
     const int col = min(c0 + tid, W - 1);
     const bool act = c0 + tid < W;
 
