@@ -287,11 +287,11 @@ int g_search_impl = BICOS_SEARCH_MFMA;
 
 // Candidate columns per LDS chunk of the full-row matrix-core searches,
 // indexed by descriptor words (1, 2, 4, 8; 0 = no such kernel). fp4: up to
-// 78 KiB of LDS (two blocks per CU); 256-bit descriptors keep 448 columns,
-// where one prefetched column per thread keeps the kernel at four waves per
-// SIMD. int8: chunks filling 64 KiB (measured best of the tried shapes for
+// 78 KiB of LDS (two blocks per CU); 256-bit descriptors take 512 columns
+// (72 KiB), one prefetched column per each of the 512 threads, which keeps
+// the kernel at four waves per SIMD (a 4096-column row is eight chunks). int8: chunks filling 64 KiB (measured best of the tried shapes for
 // 1920-wide rows of 128-bit descriptors).
-constexpr int FP4_CHUNK_COLS[9] = {0, 0, 1664, 0, 960, 0, 0, 0, 448};
+constexpr int FP4_CHUNK_COLS[9] = {0, 0, 1664, 0, 960, 0, 0, 0, 512};
 constexpr int I8_CHUNK_COLS[9] = {0, 1344, 800, 0, 448, 0, 0, 0, 224};
 
 template<int WORDS, int NB, int WAVES, int CH>
