@@ -148,6 +148,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma(
     const int16_t* __restrict__ qcol, const int* __restrict__ row_count, int16_t* __restrict__ res) {
     constexpr int KS = WORDS;              // K steps of 32 bits
     constexpr int STRIDE = 2 * KS + 1;     // uint4 per staged candidate (16 B pad)
+    static_assert(CH % 32 == 0, "chunks of whole 32-column tiles (staging rounds a chunk up to a tile)");
     constexpr int CSHIFT = EMBED ? 12 : 16;           // column field width of a key
     constexpr int CMASK = (1 << CSHIFT) - 1;
     extern __shared__ uint4 lds[];
@@ -427,6 +428,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search_mfma_fp4(
     const int16_t* __restrict__ rev, int max_lr_diff, int16_t* __restrict__ out, int64_t ostep,
     const int16_t* __restrict__ qcol, const int* __restrict__ row_count, int16_t* __restrict__ res) {
     static_assert(WORDS % 2 == 0, "fp4 form: 64-bit K steps");
+    static_assert(CH % 32 == 0, "chunks of whole 32-column tiles (staging rounds a chunk up to a tile)");
     constexpr int KS = WORDS / 2;          // K steps of 64 bits
     constexpr int STRIDE = WORDS + 1;      // uint4 per staged candidate (16 B pad)
     constexpr int SCALE_C = 127 + 7;       // E8M0 2^7 for the candidates,
